@@ -1,0 +1,233 @@
+"""ctypes bindings for the CPU oracle (oracle.c) and, when built, the compiled
+reference (oracle/_ref/libroift_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline.
+The product package (roift_amd) never imports this module.
+
+Volumes are numpy arrays of shape (zs, ys, xs) in C order, so the linear index
+is x + y*xs + z*xs*ys, gft's addressing (gft_scene32.h:173-179).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+NIL = -1
+INT32_MAX = 2**31 - 1
+
+
+def build(reference: str = "/root/reference") -> None:
+    """Compile liboracle.so and, if the reference checkout exists, _ref/."""
+    subprocess.run(["make", "-s", "-C", _HERE, "all"], check=True)
+    if os.path.isdir(os.path.join(reference, "gft", "src")):
+        subprocess.run(["make", "-s", "-j8", "-C", _HERE, "ref", f"REF={reference}"], check=True)
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        i32p, f32p, i8p = C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_int8)
+        u32p, i64p, f64p = C.POINTER(C.c_uint32), C.POINTER(C.c_int64), C.POINTER(C.c_double)
+        L.orc_spheric.argtypes = [C.c_float, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]
+        L.orc_oift_image.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, f32p,
+                                     C.c_int, i32p, i32p, i32p, i8p, i32p, i32p, i64p]
+        L.orc_edge_weights.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, u32p, u32p]
+        L.orc_oift_edges.argtypes = [u32p, u32p, C.c_int, C.c_int, C.c_int, C.c_float, i8p, C.c_int,
+                                     i32p, i32p, i32p, i8p, u32p, i32p, i64p]
+        L.orc_relax_f32.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, f32p,
+                                    C.c_int, i32p, i32p, C.c_int, C.c_int, f32p, i32p, i32p]
+        L.orc_relax_f64.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, f32p,
+                                    C.c_int, i32p, i32p, C.c_int, C.c_int, f64p, i32p, i32p]
+        L.orc_relax_binary_f32.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, i32p, i32p,
+                                           C.c_int, f32p]
+        _LIB = L
+    return _LIB
+
+
+def ref_lib():
+    """The compiled reference, or None when oracle/_ref was not built."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libroift_ref.so")
+        if not os.path.exists(path):
+            return None
+        L = C.CDLL(path)
+        i32p, f32p, f64p = C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_double)
+        L.ref_oift.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, f32p, C.c_int,
+                               i32p, i32p]
+        L.ref_relax.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, f32p, C.c_int,
+                                i32p, i32p, C.c_int]
+        L.ref_pipeline.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, i32p, i32p, C.c_int,
+                                   f64p, f64p]
+        _REF = L
+    return _REF
+
+
+def _dims(vol):
+    zs, ys, xs = vol.shape
+    return xs, ys, zs
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _gft_S(seeds):
+    s = _i32(seeds).ravel()
+    return np.concatenate([np.array([s.size], np.int32), s])
+
+
+def spheric(r: float):
+    dx = (C.c_int * 125)(); dy = (C.c_int * 125)(); dz = (C.c_int * 125)()
+    n = lib().orc_spheric(C.c_float(r), dx, dy, dz, 125)
+    assert n > 0
+    return np.array([(dx[i], dy[i], dz[i]) for i in range(n)], dtype=np.int32)
+
+
+def oift_image(I, seeds, label, adj_r=1.0, per=0.0, per_class=None, want_order=False):
+    """OIFT_Multi (per) or OIFT_Multi_PerClass (per_class) restated.
+
+    Returns dict(label, cost, pred, value[, order]).  seeds: 1-D linear indices
+    in S order (or None for the S==NULL form).
+    """
+    I = _i32(I)
+    xs, ys, zs = _dims(I)
+    lab = _i32(label).copy()
+    n = I.size
+    cost = np.empty(n, np.int32); pred = np.empty(n, np.int8); val = np.empty(n, np.int32)
+    order = np.empty(n, np.int32) if want_order else None
+    npop = np.zeros(1, np.int64)
+    S = None if seeds is None else _gft_S(seeds)
+    pc = None if per_class is None else np.ascontiguousarray(per_class, np.float32)
+    rc = lib().orc_oift_image(_p(I, C.c_int32), xs, ys, zs, adj_r, int(pc is not None), per,
+                              _p(pc, C.c_float), -1 if pc is None else pc.size - 1,
+                              _p(S, C.c_int32), _p(lab, C.c_int32), _p(cost, C.c_int32), _p(pred, C.c_int8),
+                              _p(val, C.c_int32), _p(order, C.c_int32), _p(npop, C.c_int64))
+    if rc != 0:
+        raise ValueError(f"orc_oift_image failed rc={rc}")
+    out = dict(label=lab.reshape(I.shape), cost=cost.reshape(I.shape), pred=pred.reshape(I.shape),
+               value=val.reshape(I.shape))
+    if want_order:
+        out["order"] = order[: int(npop[0])]
+    return out
+
+
+def edge_weights(I, adj_r=1.0, per=0.0):
+    """(wa, wb) of shape (E, zs, ys, xs): wa = w_obj(p->p+d), wb = w_obj(p+d->p)."""
+    I = _i32(I)
+    xs, ys, zs = _dims(I)
+    E = (spheric(adj_r).shape[0] - 1) // 2
+    wa = np.empty((E,) + I.shape, np.uint32); wb = np.empty((E,) + I.shape, np.uint32)
+    rc = lib().orc_edge_weights(_p(I, C.c_int32), xs, ys, zs, adj_r, per, _p(wa, C.c_uint32), _p(wb, C.c_uint32))
+    assert rc == 0
+    return wa, wb
+
+
+def oift_edges(wa, wb, seeds, label, adj_r=1.0, pol_sign=None, want_order=False):
+    wa = np.ascontiguousarray(wa, np.uint32); wb = np.ascontiguousarray(wb, np.uint32)
+    shape = wa.shape[1:]
+    zs, ys, xs = shape
+    lab = _i32(label).copy()
+    n = lab.size
+    cost = np.empty(n, np.int32); pred = np.empty(n, np.int8); val = np.empty(n, np.uint32)
+    order = np.empty(n, np.int32) if want_order else None
+    npop = np.zeros(1, np.int64)
+    S = None if seeds is None else _gft_S(seeds)
+    ps = None if pol_sign is None else np.ascontiguousarray(pol_sign, np.int8)
+    rc = lib().orc_oift_edges(_p(wa, C.c_uint32), _p(wb, C.c_uint32), xs, ys, zs, adj_r, _p(ps, C.c_int8),
+                              -1 if ps is None else ps.size - 1, _p(S, C.c_int32), _p(lab, C.c_int32),
+                              _p(cost, C.c_int32), _p(pred, C.c_int8), _p(val, C.c_uint32),
+                              _p(order, C.c_int32), _p(npop, C.c_int64))
+    if rc != 0:
+        raise ValueError(f"orc_oift_edges failed rc={rc}")
+    out = dict(label=lab.reshape(shape), cost=cost.reshape(shape), pred=pred.reshape(shape),
+               value=val.reshape(shape))
+    if want_order:
+        out["order"] = order[: int(npop[0])]
+    return out
+
+
+def relax(I, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None, fp64=False, dense=False):
+    """ORelax_1_Multi[_PerClass] restated.  Returns dict(label, memb[nc, ...], classes)."""
+    I = _i32(I)
+    xs, ys, zs = _dims(I)
+    lab = _i32(label).copy()
+    S = _gft_S(seeds)
+    pc = None if per_class is None else np.ascontiguousarray(per_class, np.float32)
+    nc = np.zeros(1, np.int32); cls = np.zeros(64, np.int32)
+    # memberships for at most 64 classes; sized after counting classes below
+    present = {0} | {int(lab.ravel()[s]) for s in np.asarray(seeds).ravel() if lab.ravel()[s] >= 0}
+    ncl = len(present)
+    memb = np.empty((ncl,) + I.shape, np.float64 if fp64 else np.float32)
+    fn = lib().orc_relax_f64 if fp64 else lib().orc_relax_f32
+    rc = fn(_p(I, C.c_int32), xs, ys, zs, adj_r, int(pc is not None), per, _p(pc, C.c_float),
+            -1 if pc is None else pc.size - 1, _p(S, C.c_int32), _p(lab, C.c_int32), int(niter), int(dense),
+            _p(memb, C.c_double if fp64 else C.c_float), _p(cls, C.c_int32), _p(nc, C.c_int32))
+    if rc != 0:
+        raise ValueError(f"orc_relax failed rc={rc}")
+    assert nc[0] == ncl
+    return dict(label=lab.reshape(I.shape), memb=memb, classes=cls[: nc[0]].copy())
+
+
+def relax_binary(I, seeds, label, niter, adj_r=1.0, per=0.0):
+    I = _i32(I)
+    xs, ys, zs = _dims(I)
+    lab = _i32(label).copy()
+    S = None if seeds is None else _gft_S(seeds)
+    memb = np.empty(I.shape, np.float32)
+    rc = lib().orc_relax_binary_f32(_p(I, C.c_int32), xs, ys, zs, adj_r, per, _p(S, C.c_int32),
+                                    _p(lab, C.c_int32), int(niter), _p(memb, C.c_float))
+    assert rc == 0
+    return dict(label=lab.reshape(I.shape), memb=memb)
+
+
+# ---- the compiled reference (only where oracle/_ref was built) -------------------
+
+def ref_oift(I, seeds, label, adj_r=1.0, per=0.0, per_class=None):
+    L = ref_lib()
+    I = _i32(I); xs, ys, zs = _dims(I)
+    lab = _i32(label).copy()
+    S = None if seeds is None else _gft_S(seeds)
+    pc = None if per_class is None else np.ascontiguousarray(per_class, np.float32)
+    L.ref_oift(_p(I, C.c_int32), xs, ys, zs, adj_r, int(pc is not None), per, _p(pc, C.c_float),
+               -1 if pc is None else pc.size - 1, _p(S, C.c_int32), _p(lab, C.c_int32))
+    return lab
+
+
+def ref_relax(I, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None):
+    L = ref_lib()
+    I = _i32(I); xs, ys, zs = _dims(I)
+    lab = _i32(label).copy()
+    S = _gft_S(seeds)
+    pc = None if per_class is None else np.ascontiguousarray(per_class, np.float32)
+    L.ref_relax(_p(I, C.c_int32), xs, ys, zs, adj_r, int(pc is not None), per, _p(pc, C.c_float),
+                -1 if pc is None else pc.size - 1, _p(S, C.c_int32), _p(lab, C.c_int32), int(niter))
+    return lab
+
+
+def ref_pipeline(I, seeds, label, niter, adj_r=1.0, per=0.0):
+    """Reference OIFT_Multi + ORelax_1_Multi; returns (labels, t_oift_s, t_relax_s)."""
+    L = ref_lib()
+    I = _i32(I); xs, ys, zs = _dims(I)
+    lab = _i32(label).copy()
+    S = _gft_S(seeds)
+    t1 = np.zeros(1); t2 = np.zeros(1)
+    L.ref_pipeline(_p(I, C.c_int32), xs, ys, zs, adj_r, per, _p(S, C.c_int32), _p(lab, C.c_int32),
+                   int(niter), _p(t1, C.c_double), _p(t2, C.c_double))
+    return lab, float(t1[0]), float(t2[0])

@@ -1,0 +1,666 @@
+/*
+ * oracle.c -- CPU restatement of roift's OIFT (f_max) + fuzzy relaxation hot path.
+ *
+ * THIS IS TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it, and only as the checker / the
+ * timed CPU baseline.  The product path (roift_amd/, libroift.so) never links
+ * or calls anything in this directory.
+ *
+ * Every function restates the reference algorithm from its published source
+ * (file:line citations are into jung0221/roift) in code written for this
+ * project, and additionally emits the quantities the reference computes but
+ * discards (the f_max cost map, the predecessor direction, the pop order).
+ *
+ * Parity pins (what checks each function; see tests/test_oracle_*.py):
+ *   orc_spheric          -- neighbour order vs gft_adjrel3.cpp:47-85 (closed form)
+ *   orc_oift_image       -- golden fixtures produced by the compiled reference
+ *                           (tests/golden), oracle/_ref when built, Kruskal
+ *                           minimax closed form for per=0, brute-force optimum
+ *                           on tiny graphs, forest/Bellman invariants.
+ *   orc_oift_edges       -- equality with orc_oift_image on image-derived edge
+ *                           weights (two different priority queues), invariants.
+ *   orc_edge_weights     -- exhaustive identity check against the closed form.
+ *   orc_relax            -- golden fixtures (labels), oracle/_ref (labels),
+ *                           closed forms on constant images, invariants,
+ *                           masked == dense equality.
+ *   orc_relax_binary     -- oracle/_ref labels when built; otherwise "parity
+ *                           unpinned" beyond the constant-image closed form.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <limits.h>
+#include <math.h>
+
+#define ORC_NIL (-1)
+#define ORC_WHITE 0
+#define ORC_GRAY 1
+#define ORC_BLACK 2
+
+enum { ORC_OK = 0, ORC_EINVAL = -1, ORC_ENOMEM = -2 };
+
+/* ------------------------------------------------------------------------ */
+/* Adjacency: gft::AdjRel3::Spheric(r) (gft_adjrel3.cpp:47-85).              */
+/* Raster order dz, dy, dx over the ball r^2 >= dx^2+dy^2+dz^2 with the      */
+/* centre moved to index 0 (the others keep raster order).                   */
+/* ------------------------------------------------------------------------ */
+int orc_spheric(float r, int *dx, int *dy, int *dz, int cap)
+{
+    int r0 = (int)r, r2 = (int)(r * r);
+    int n = 0, i0 = 0;
+    for (int z = -r0; z <= r0; z++)
+        for (int y = -r0; y <= r0; y++)
+            for (int x = -r0; x <= r0; x++)
+                if (x * x + y * y + z * z <= r2) {
+                    if (n >= cap) return ORC_EINVAL;
+                    dx[n] = x; dy[n] = y; dz[n] = z;
+                    if (x == 0 && y == 0 && z == 0) i0 = n;
+                    n++;
+                }
+    /* rotate the centre to the front */
+    int cx = dx[i0], cy = dy[i0], cz = dz[i0];
+    for (int i = i0; i > 0; i--) { dx[i] = dx[i - 1]; dy[i] = dy[i - 1]; dz[i] = dz[i - 1]; }
+    dx[0] = cx; dy[0] = cy; dz[0] = cz;
+    return n;
+}
+
+typedef struct {
+    int n;          /* A->n including the centre */
+    int dx[125], dy[125], dz[125];
+} orc_adj;
+
+static int adj_make(float r, orc_adj *A)
+{
+    A->n = orc_spheric(r, A->dx, A->dy, A->dz, 125);
+    return A->n > 0 ? ORC_OK : ORC_EINVAL;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Oriented arc weight, OIFT_Multi (gft_ift.cpp:965-973) and the PerClass    */
+/* variant (gft_ift.cpp:1055-1067): |I_p-I_q| scaled by (1 +- per_pq/100) in */
+/* double and truncated toward zero by the int assignment.                   */
+/* ------------------------------------------------------------------------ */
+static inline int arc_weight(int Ip, int Iq, float per_pq)
+{
+    int w = abs(Ip - Iq);
+    if (Ip > Iq)
+        w = (int)((double)w * (1.0 + (double)per_pq / 100.0));
+    else if (Ip < Iq)
+        w = (int)((double)w * (1.0 - (double)per_pq / 100.0));
+    return w;
+}
+
+static inline float label_polarity(int lb, int use_pc, float per, const float *pc, int max_label)
+{
+    if (!use_pc)
+        return (lb == 0) ? -per : per;          /* gft_ift.cpp:968 */
+    if (lb >= 0 && lb <= max_label)             /* gft_ift.cpp:1058-1062 */
+        return pc[lb];
+    return 0.0f;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Dial bucket queue with global-min FIFO pops, restating gft PQueue32's      */
+/* Fast* operations (gft_pqueue32.cpp:231-245, 300-327, 353-375, 415-420,    */
+/* 446-451): insert appends at the bucket tail and lowers minvalue; the pop  */
+/* scans upward from minvalue and takes the bucket head.                     */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int *first, *last;   /* per bucket */
+    int *next, *prev;    /* per element */
+    char *color;
+    int nbuckets, minvalue, nadded;
+    const int *value;
+} dialq;
+
+static int dq_create(dialq *Q, int nbuckets, int nelems, const int *value)
+{
+    Q->first = (int *)malloc(sizeof(int) * (size_t)(nbuckets + 1));
+    Q->last = (int *)malloc(sizeof(int) * (size_t)(nbuckets + 1));
+    Q->next = (int *)malloc(sizeof(int) * (size_t)nelems);
+    Q->prev = (int *)malloc(sizeof(int) * (size_t)nelems);
+    Q->color = (char *)malloc((size_t)nelems);
+    if (!Q->first || !Q->last || !Q->next || !Q->prev || !Q->color) return ORC_ENOMEM;
+    for (int b = 0; b <= nbuckets; b++) Q->first[b] = Q->last[b] = ORC_NIL;
+    for (int e = 0; e < nelems; e++) { Q->next[e] = Q->prev[e] = ORC_NIL; Q->color[e] = ORC_WHITE; }
+    Q->nbuckets = nbuckets; Q->minvalue = INT_MAX; Q->nadded = 0; Q->value = value;
+    return ORC_OK;
+}
+
+static void dq_destroy(dialq *Q)
+{
+    free(Q->first); free(Q->last); free(Q->next); free(Q->prev); free(Q->color);
+}
+
+static inline void dq_insert(dialq *Q, int e)
+{
+    int b = Q->value[e];
+    Q->nadded++;
+    if (b < Q->minvalue) Q->minvalue = b;
+    if (Q->first[b] == ORC_NIL) { Q->first[b] = e; Q->prev[e] = ORC_NIL; }
+    else { Q->next[Q->last[b]] = e; Q->prev[e] = Q->last[b]; }
+    Q->last[b] = e;
+    Q->next[e] = ORC_NIL;
+    Q->color[e] = ORC_GRAY;
+}
+
+static inline void dq_remove(dialq *Q, int e)
+{
+    int b = Q->value[e], pv = Q->prev[e], nx = Q->next[e];
+    Q->nadded--;
+    if (Q->first[b] == e) {
+        Q->first[b] = nx;
+        if (nx == ORC_NIL) Q->last[b] = ORC_NIL; else Q->prev[nx] = ORC_NIL;
+    } else {
+        Q->next[pv] = nx;
+        if (nx == ORC_NIL) Q->last[b] = pv; else Q->prev[nx] = pv;
+    }
+    Q->color[e] = ORC_BLACK;
+}
+
+static inline int dq_pop_min_fifo(dialq *Q)
+{
+    Q->nadded--;
+    while (Q->first[Q->minvalue] == ORC_NIL) Q->minvalue++;
+    int b = Q->minvalue, e = Q->first[b], nx = Q->next[e];
+    if (nx == ORC_NIL) Q->first[b] = Q->last[b] = ORC_NIL;
+    else { Q->first[b] = nx; Q->prev[nx] = ORC_NIL; }
+    Q->color[e] = ORC_BLACK;
+    return e;
+}
+
+/* ------------------------------------------------------------------------ */
+/* OIFT_Multi / OIFT_Multi_PerClass (gft_ift.cpp:911-989, 991-1083).         */
+/*                                                                            */
+/* label:  in/out, NIL=-1 for unlabelled voxels (as the reference).           */
+/* S:      gft seed array, S[0]=count, S[1..] linear indices; NULL inserts    */
+/*         every labelled voxel in raster order (gft_ift.cpp:944-948).        */
+/* cost:   (optional) f_max cost C(q)=max(C(pred),value(q)) assigned at pop;  */
+/*         INT32_MAX for voxels never popped.                                 */
+/* pred:   (optional) adjacency index i (q = pred + d_i) of the last          */
+/*         successful relaxation, 0 for seeds and never-popped voxels.        */
+/* value:  (optional) final last-arc value (INT_MAX / 0 as initialised).      */
+/* order:  (optional) voxels in pop order; *npop receives the count.          */
+/* ------------------------------------------------------------------------ */
+int orc_oift_image(const int32_t *I, int xs, int ys, int zs, float adj_r,
+                   int use_per_class, float per, const float *per_class, int max_label,
+                   const int32_t *S, int32_t *label,
+                   int32_t *cost, int8_t *pred, int32_t *value_out,
+                   int32_t *order, int64_t *npop)
+{
+    orc_adj A;
+    if (adj_make(adj_r, &A) != ORC_OK) return ORC_EINVAL;
+    if (xs <= 0 || ys <= 0 || zs <= 0) return ORC_EINVAL;
+    const int64_t n64 = (int64_t)xs * ys * zs;
+    if (n64 > INT_MAX) return ORC_EINVAL;
+    const int n = (int)n64;
+    int Imax = INT_MIN, Imin = INT_MAX;
+    for (int p = 0; p < n; p++) { if (I[p] > Imax) Imax = I[p]; if (I[p] < Imin) Imin = I[p]; }
+    if (Imin < 0) return ORC_EINVAL; /* precondition of the bucket range (oiftrelax.cpp:263-268) */
+
+    float max_abs = fabsf(per);
+    if (use_per_class) {
+        if (!per_class || max_label < 0) return ORC_EINVAL;
+        max_abs = 0.0f;
+        for (int c = 0; c <= max_label; c++) if (fabsf(per_class[c]) > max_abs) max_abs = fabsf(per_class[c]);
+    }
+    if (max_abs > 100.0f) return ORC_EINVAL;
+    /* int Wmax *= double  (gft_ift.cpp:925-926, 1023-1024) */
+    int Wmax = (int)((double)Imax * (1.0 + (double)max_abs / 100.0));
+
+    if (S) {
+        for (int i = 1; i <= S[0]; i++) {
+            if (S[i] < 0 || S[i] >= n) return ORC_EINVAL;
+            if (label[S[i]] == ORC_NIL) return ORC_EINVAL; /* would index bucket INT_MAX in the reference */
+        }
+    }
+
+    int *value = (int *)malloc(sizeof(int) * (size_t)n);
+    int32_t *C = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    dialq Q = {0};
+    if (!value || !C || dq_create(&Q, Wmax + 2, n, value) != ORC_OK) {
+        free(value); free(C); dq_destroy(&Q); return ORC_ENOMEM;
+    }
+    for (int p = 0; p < n; p++) {
+        value[p] = (label[p] == ORC_NIL) ? INT_MAX : 0;   /* gft_ift.cpp:930-936 */
+        C[p] = INT32_MAX;
+        if (pred) pred[p] = 0;
+    }
+    if (S) {
+        for (int i = 1; i <= S[0]; i++) dq_insert(&Q, S[i]);
+    } else {
+        for (int p = 0; p < n; p++) if (label[p] != ORC_NIL) dq_insert(&Q, p);
+    }
+
+    const int xy = xs * ys;
+    int64_t k = 0;
+    int level = 0;
+    while (Q.nadded > 0) {
+        int p = dq_pop_min_fifo(&Q);
+        /* cost at pop = running flood level = max(C(pred), value(p)) */
+        if (value[p] > level) level = value[p];
+        C[p] = level;
+        if (order) order[k] = p;
+        k++;
+        const int px = (p % xy) % xs, py = (p % xy) / xs, pz = p / xy;
+        const int lp = label[p];
+        const float per_pq = label_polarity(lp, use_per_class, per, per_class, max_label);
+        for (int i = 1; i < A.n; i++) {
+            int qx = px + A.dx[i], qy = py + A.dy[i], qz = pz + A.dz[i];
+            if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) continue;
+            int q = qx + qy * xs + qz * xy;
+            if (Q.color[q] == ORC_BLACK) continue;
+            int w = arc_weight(I[p], I[q], per_pq);
+            if (w < value[q]) {
+                if (Q.color[q] == ORC_GRAY) dq_remove(&Q, q);
+                value[q] = w;
+                label[q] = lp;
+                if (pred) pred[q] = (int8_t)i;
+                dq_insert(&Q, q);
+            }
+        }
+    }
+    if (cost) memcpy(cost, C, sizeof(int32_t) * (size_t)n);
+    if (value_out) memcpy(value_out, value, sizeof(int32_t) * (size_t)n);
+    if (npop) *npop = k;
+    free(value); free(C); dq_destroy(&Q);
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Oriented edge weights for a two-polarity problem (OIFT_Multi form):       */
+/* for every forward neighbour slot s (adjacency index i = (n-1)/2+1+s,      */
+/* q = p + d_i) store                                                         */
+/*     wa[s*N+p] = w_obj(p->q),   wb[s*N+p] = w_obj(q->p)                    */
+/* where w_obj uses per_pq=+per (labels != 0) and w_bg(p->q) = w_obj(q->p).  */
+/* Slots whose q lies outside the volume hold 0xFFFFFFFF.                    */
+/* ------------------------------------------------------------------------ */
+int orc_edge_weights(const int32_t *I, int xs, int ys, int zs, float adj_r, float per,
+                     uint32_t *wa, uint32_t *wb)
+{
+    orc_adj A;
+    if (adj_make(adj_r, &A) != ORC_OK) return ORC_EINVAL;
+    const int half = (A.n - 1) / 2;
+    const int64_t n = (int64_t)xs * ys * zs;
+    const int xy = xs * ys;
+    for (int s = 0; s < half; s++) {
+        int i = half + 1 + s;
+        for (int64_t p = 0; p < n; p++) {
+            int px = (int)((p % xy) % xs), py = (int)((p % xy) / xs), pz = (int)(p / xy);
+            int qx = px + A.dx[i], qy = py + A.dy[i], qz = pz + A.dz[i];
+            if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) {
+                wa[s * n + p] = 0xFFFFFFFFu; wb[s * n + p] = 0xFFFFFFFFu; continue;
+            }
+            int64_t q = qx + (int64_t)qy * xs + (int64_t)qz * xy;
+            wa[s * n + p] = (uint32_t)arc_weight(I[p], I[q], per);
+            wb[s * n + p] = (uint32_t)arc_weight(I[q], I[p], per);
+        }
+    }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The same Prim-style OIFT driven by explicit oriented edge weights (wa,wb  */
+/* as produced above, or tie-free ranks), with a lazy binary heap keyed by   */
+/* (value, insertion sequence).  Popping the minimum (value, seq) is the      */
+/* Dial queue's global-min FIFO order, so this equals orc_oift_image on       */
+/* image-derived weights while supporting weights up to 2^32-2.              */
+/* Labels with pol_sign[label]>0 use w_obj, the others w_bg; labels outside  */
+/* [0,max_label] use w_obj (as OIFT_Multi treats every label != 0).          */
+/* ------------------------------------------------------------------------ */
+typedef struct { uint64_t key; int32_t v; } heap_ent; /* key = value<<32 | seq_lo */
+
+typedef struct { heap_ent *a; int64_t n, cap; } bheap;
+
+static int bh_push(bheap *h, uint64_t key, int32_t v)
+{
+    if (h->n == h->cap) {
+        int64_t nc = h->cap ? h->cap * 2 : 1024;
+        heap_ent *na = (heap_ent *)realloc(h->a, sizeof(heap_ent) * (size_t)nc);
+        if (!na) return ORC_ENOMEM;
+        h->a = na; h->cap = nc;
+    }
+    int64_t i = h->n++;
+    while (i > 0) {
+        int64_t par = (i - 1) / 2;
+        if (h->a[par].key <= key) break;
+        h->a[i] = h->a[par]; i = par;
+    }
+    h->a[i].key = key; h->a[i].v = v;
+    return ORC_OK;
+}
+
+static heap_ent bh_pop(bheap *h)
+{
+    heap_ent top = h->a[0], last = h->a[--h->n];
+    int64_t i = 0;
+    for (;;) {
+        int64_t c = 2 * i + 1;
+        if (c >= h->n) break;
+        if (c + 1 < h->n && h->a[c + 1].key < h->a[c].key) c++;
+        if (h->a[c].key >= last.key) break;
+        h->a[i] = h->a[c]; i = c;
+    }
+    if (h->n > 0) h->a[i] = last;
+    return top;
+}
+
+int orc_oift_edges(const uint32_t *wa, const uint32_t *wb, int xs, int ys, int zs, float adj_r,
+                   const int8_t *pol_sign, int max_label,
+                   const int32_t *S, int32_t *label,
+                   int32_t *cost, int8_t *pred, uint32_t *value_out,
+                   int32_t *order, int64_t *npop)
+{
+    orc_adj A;
+    if (adj_make(adj_r, &A) != ORC_OK) return ORC_EINVAL;
+    const int half = (A.n - 1) / 2;
+    const int64_t n64 = (int64_t)xs * ys * zs;
+    if (n64 > INT_MAX || n64 <= 0) return ORC_EINVAL;
+    const int n = (int)n64;
+    if (S) for (int i = 1; i <= S[0]; i++) {
+        if (S[i] < 0 || S[i] >= n || label[S[i]] == ORC_NIL) return ORC_EINVAL;
+    }
+    /* 32-bit insertion sequence is enough: each successful relaxation inserts once */
+    uint32_t *value = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n);
+    uint32_t *seq = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n);
+    char *color = (char *)calloc((size_t)n, 1);
+    bheap h = {0, 0, 0};
+    if (!value || !seq || !color) { free(value); free(seq); free(color); return ORC_ENOMEM; }
+    uint64_t nseq = 0;
+    int rc = ORC_OK;
+    for (int p = 0; p < n; p++) {
+        value[p] = (label[p] == ORC_NIL) ? 0xFFFFFFFFu : 0u;
+        if (cost) cost[p] = INT32_MAX;
+        if (pred) pred[p] = 0;
+    }
+#define PUSH(q) do { seq[q] = (uint32_t)nseq; \
+        if (bh_push(&h, ((uint64_t)value[q] << 32) | (uint64_t)(uint32_t)nseq, q) != ORC_OK) { rc = ORC_ENOMEM; goto done; } \
+        nseq++; color[q] = ORC_GRAY; } while (0)
+    if (S) { for (int i = 1; i <= S[0]; i++) PUSH(S[i]); }
+    else { for (int p = 0; p < n; p++) if (label[p] != ORC_NIL) PUSH(p); }
+
+    const int xy = xs * ys;
+    int64_t k = 0;
+    uint32_t level = 0;
+    while (h.n > 0) {
+        heap_ent e = bh_pop(&h);
+        int p = e.v;
+        if (color[p] == ORC_BLACK || (uint32_t)e.key != seq[p] || (uint32_t)(e.key >> 32) != value[p]) continue;
+        color[p] = ORC_BLACK;
+        if (value[p] > level) level = value[p];
+        if (cost) cost[p] = (int32_t)level;
+        if (order) order[k] = p;
+        k++;
+        const int px = (p % xy) % xs, py = (p % xy) / xs, pz = p / xy;
+        const int lp = label[p];
+        const int obj = (lp >= 0 && lp <= max_label && pol_sign) ? (pol_sign[lp] > 0) : (lp != 0);
+        for (int i = 1; i < A.n; i++) {
+            int qx = px + A.dx[i], qy = py + A.dy[i], qz = pz + A.dz[i];
+            if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) continue;
+            int q = qx + qy * xs + qz * xy;
+            if (color[q] == ORC_BLACK) continue;
+            uint32_t w;
+            if (i > half) { int s = i - half - 1; w = obj ? wa[(int64_t)s * n + p] : wb[(int64_t)s * n + p]; }
+            else { int s = (A.n - i) - half - 1; w = obj ? wb[(int64_t)s * n + q] : wa[(int64_t)s * n + q]; }
+            if (w < value[q]) {
+                value[q] = w;
+                label[q] = lp;
+                if (pred) pred[q] = (int8_t)i;
+                PUSH(q);
+            }
+        }
+    }
+#undef PUSH
+    if (value_out) memcpy(value_out, value, sizeof(uint32_t) * (size_t)n);
+    if (npop) *npop = k;
+done:
+    free(h.a); free(value); free(seq); free(color);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ORelax_1_Multi / ORelax_1_Multi_PerClass (gft_ift.cpp:8655-8911,          */
+/* 8913-9187), restated for float (the reference's arithmetic, operation by  */
+/* operation) and double (the FP64 oracle).                                   */
+/*                                                                            */
+/* label:   in = OIFT labels, out = argmax labels (as the reference).         */
+/* memb:    (optional) final memberships, planar [nclasses][N].               */
+/* classes: (optional) class_labels (up to 64); *nclasses_out its count.      */
+/* dense:   1 updates every voxel every iteration instead of the dilating    */
+/*          band (bit-identical, used to test that claim).                    */
+/* ------------------------------------------------------------------------ */
+#define RELAX_BODY(REAL, FABS, SQRT)                                                          \
+    orc_adj A;                                                                                \
+    if (adj_make(adj_r, &A) != ORC_OK) return ORC_EINVAL;                                     \
+    const int64_t n64 = (int64_t)xs * ys * zs;                                                \
+    if (n64 > INT_MAX || n64 <= 0) return ORC_EINVAL;                                         \
+    const int n = (int)n64;                                                                   \
+    if (!S || S[0] <= 0) return ORC_EINVAL;                                                   \
+    int max_seed_label = 0;                                                                   \
+    for (int i = 1; i <= S[0]; i++) {                                                         \
+        int p = S[i];                                                                         \
+        if (p >= 0 && p < n && label[p] > max_seed_label) max_seed_label = label[p];          \
+    }                                                                                         \
+    char *present = (char *)calloc((size_t)max_seed_label + 1, 1);                            \
+    int *l2c = (int *)malloc(sizeof(int) * ((size_t)max_seed_label + 1));                    \
+    int cls[64];                                                                              \
+    int nc = 0;                                                                               \
+    present[0] = 1;                                                                           \
+    for (int i = 1; i <= S[0]; i++) {                                                         \
+        int p = S[i];                                                                         \
+        if (p >= 0 && p < n) { int lb = label[p]; if (lb >= 0 && lb <= max_seed_label) present[lb] = 1; } \
+    }                                                                                         \
+    for (int i = 0; i <= max_seed_label; i++) {                                               \
+        l2c[i] = -1;                                                                          \
+        if (present[i]) { if (nc >= 64) { free(present); free(l2c); return ORC_EINVAL; } l2c[i] = nc; cls[nc++] = i; } \
+    }                                                                                         \
+    int bg = l2c[0] < 0 ? 0 : l2c[0];                                                         \
+    REAL pcm[64];                                                                             \
+    float max_abs = FABS(per);                                                                \
+    if (use_per_class) {                                                                      \
+        max_abs = 0.0f;                                                                       \
+        for (int c = 0; c < nc; c++) {                                                        \
+            pcm[c] = (cls[c] <= max_label) ? (REAL)per_class[cls[c]] : (REAL)0;               \
+            float a = (cls[c] <= max_label) ? fabsf(per_class[cls[c]]) : 0.0f;                \
+            if (a > max_abs) max_abs = a;                                                     \
+        }                                                                                     \
+    }                                                                                         \
+    REAL Dpq[125];                                                                            \
+    for (int i = 1; i < A.n; i++) {                                                           \
+        /* spacing dx=dy=dz=1 (Scene32::Create defaults, gft_scene32.cpp:28-30) */            \
+        REAL ax = (REAL)(A.dx[i] * A.dx[i]), ay = (REAL)(A.dy[i] * A.dy[i]), az = (REAL)(A.dz[i] * A.dz[i]); \
+        Dpq[i] = SQRT(ax * (REAL)1 * (REAL)1 + ay * (REAL)1 * (REAL)1 + az * (REAL)1 * (REAL)1) / (REAL)1; \
+    }                                                                                         \
+    int Imax = INT_MIN;                                                                       \
+    for (int p = 0; p < n; p++) if (I[p] > Imax) Imax = I[p];                                  \
+    REAL Wmax = (REAL)((double)(float)Imax * (1.0 + (double)max_abs / 100.0));                \
+    const size_t total = (size_t)n * (size_t)nc;                                              \
+    REAL *f1 = (REAL *)calloc(total, sizeof(REAL));                                           \
+    REAL *f2 = (REAL *)calloc(total, sizeof(REAL));                                           \
+    int *mnodes = (int *)malloc(sizeof(int) * ((size_t)n + 1));                               \
+    char *mask = (char *)calloc((size_t)n, 1);                                                \
+    if (!f1 || !f2 || !mnodes || !mask) { free(f1); free(f2); free(mnodes); free(mask); free(present); free(l2c); return ORC_ENOMEM; } \
+    const int xy = xs * ys;                                                                   \
+    mnodes[0] = 0;                                                                            \
+    for (int p = 0; p < n; p++) {                                                             \
+        int lb = label[p], c = bg;                                                            \
+        if (lb >= 0 && lb <= max_seed_label && l2c[lb] >= 0) c = l2c[lb];                     \
+        f1[(size_t)p * nc + c] = (REAL)1;                                                     \
+        int px = (p % xy) % xs, py = (p % xy) / xs, pz = p / xy;                               \
+        for (int i = 1; i < A.n; i++) {                                                       \
+            int qx = px + A.dx[i], qy = py + A.dy[i], qz = pz + A.dz[i];                      \
+            if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) continue;     \
+            int q = qx + qy * xs + qz * xy;                                                   \
+            if (label[p] != label[q]) { mask[p] = 1; mnodes[++mnodes[0]] = p; break; }        \
+        }                                                                                     \
+    }                                                                                         \
+    if (dense) { mnodes[0] = n; for (int p = 0; p < n; p++) { mnodes[p + 1] = p; mask[p] = 1; } } \
+    int ninic = 1;                                                                            \
+    while (ntimes > 0) {                                                                      \
+        memcpy(f2, f1, total * sizeof(REAL));                                                 \
+        for (int j = 1; j <= mnodes[0]; j++) {                                                \
+            int p = mnodes[j];                                                                \
+            int px = (p % xy) % xs, py = (p % xy) / xs, pz = p / xy;                           \
+            size_t po = (size_t)p * nc;                                                       \
+            for (int k = 0; k < nc; k++) f2[po + k] = (REAL)0;                                \
+            REAL sw = (REAL)0;                                                                \
+            for (int i = 1; i < A.n; i++) {                                                   \
+                int qx = px + A.dx[i], qy = py + A.dy[i], qz = pz + A.dz[i];                  \
+                if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) continue; \
+                int q = qx + qy * xs + qz * xy;                                               \
+                size_t qo = (size_t)q * nc;                                                   \
+                REAL w = FABS((REAL)(I[p] - I[q]));                                           \
+                REAL s;                                                                       \
+                if (use_per_class) { s = (REAL)0; for (int k = 0; k < nc; k++) s += pcm[k] * f1[qo + k]; } \
+                else s = (REAL)per * ((REAL)2 * f1[qo + bg] - (REAL)1);                       \
+                if (I[p] > I[q]) w *= ((REAL)1 + s / (REAL)100);                               \
+                else if (I[p] < I[q]) w *= ((REAL)1 - s / (REAL)100);                          \
+                w = Wmax - w;                                                                 \
+                if (w < (REAL)0) w = (REAL)0;                                                 \
+                w = w * w; w = w * w; w = w * w;                                              \
+                w /= Dpq[i];                                                                  \
+                sw += w;                                                                      \
+                for (int k = 0; k < nc; k++) f2[po + k] += w * f1[qo + k];                    \
+            }                                                                                 \
+            if (sw > (REAL)0) { for (int k = 0; k < nc; k++) f2[po + k] /= sw; }              \
+            else { for (int k = 0; k < nc; k++) f2[po + k] = f1[po + k]; }                    \
+        }                                                                                     \
+        REAL *t = f1; f1 = f2; f2 = t;                                                        \
+        for (int i = 1; i <= S[0]; i++) {                                                     \
+            int p = S[i];                                                                     \
+            if (p < 0 || p >= n) continue;                                                    \
+            int lb = label[p], c = bg;                                                        \
+            if (lb >= 0 && lb <= max_seed_label && l2c[lb] >= 0) c = l2c[lb];                 \
+            size_t po = (size_t)p * nc;                                                       \
+            for (int k = 0; k < nc; k++) f1[po + k] = (REAL)0;                                \
+            f1[po + c] = (REAL)1;                                                             \
+        }                                                                                     \
+        ntimes--;                                                                             \
+        if (ntimes > 0 && !dense) {                                                           \
+            int nlast = mnodes[0];                                                            \
+            for (int j = ninic; j <= mnodes[0]; j++) {                                        \
+                int p = mnodes[j];                                                            \
+                int px = (p % xy) % xs, py = (p % xy) / xs, pz = p / xy;                       \
+                for (int i = 1; i < A.n; i++) {                                               \
+                    int qx = px + A.dx[i], qy = py + A.dy[i], qz = pz + A.dz[i];              \
+                    if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) continue; \
+                    int q = qx + qy * xs + qz * xy;                                           \
+                    if (!mask[q]) { mask[q] = 1; mnodes[++nlast] = q; }                       \
+                }                                                                             \
+            }                                                                                 \
+            ninic = mnodes[0] + 1;                                                            \
+            mnodes[0] = nlast;                                                                \
+        }                                                                                     \
+    }                                                                                         \
+    for (int p = 0; p < n; p++) {                                                             \
+        size_t po = (size_t)p * nc;                                                           \
+        int best = 0;                                                                         \
+        REAL bv = f1[po];                                                                     \
+        for (int k = 1; k < nc; k++) if (f1[po + k] > bv) { bv = f1[po + k]; best = k; }      \
+        label[p] = cls[best];                                                                 \
+    }                                                                                         \
+    if (memb) for (int k = 0; k < nc; k++) for (int p = 0; p < n; p++)                        \
+        memb[(size_t)k * n + p] = f1[(size_t)p * nc + k];                                     \
+    if (classes) for (int k = 0; k < nc; k++) classes[k] = cls[k];                            \
+    if (nclasses_out) *nclasses_out = nc;                                                     \
+    free(f1); free(f2); free(mnodes); free(mask); free(present); free(l2c);                   \
+    return ORC_OK;
+
+int orc_relax_f32(const int32_t *I, int xs, int ys, int zs, float adj_r,
+                  int use_per_class, float per, const float *per_class, int max_label,
+                  const int32_t *S, int32_t *label, int ntimes, int dense,
+                  float *memb, int32_t *classes, int32_t *nclasses_out)
+{
+    RELAX_BODY(float, fabsf, sqrtf)
+}
+
+int orc_relax_f64(const int32_t *I, int xs, int ys, int zs, float adj_r,
+                  int use_per_class, float per, const float *per_class, int max_label,
+                  const int32_t *S, int32_t *label, int ntimes, int dense,
+                  double *memb, int32_t *classes, int32_t *nclasses_out)
+{
+    RELAX_BODY(double, fabs, sqrt)
+}
+
+/* ------------------------------------------------------------------------ */
+/* ORelax_1 binary fallback (gft_ift.cpp:8498-8653): one channel            */
+/* f = label/Imax, hard polarity from f(q) < 0.5, no clamp, int Wmax.        */
+/* ORelax_1_Multi routes here when S is NULL or empty (gft_ift.cpp:8678).    */
+/* ------------------------------------------------------------------------ */
+int orc_relax_binary_f32(const int32_t *I, int xs, int ys, int zs, float adj_r, float per,
+                         const int32_t *S, int32_t *label, int ntimes, float *memb)
+{
+    orc_adj A;
+    if (adj_make(adj_r, &A) != ORC_OK) return ORC_EINVAL;
+    const int64_t n64 = (int64_t)xs * ys * zs;
+    if (n64 > INT_MAX || n64 <= 0) return ORC_EINVAL;
+    const int n = (int)n64, xy = xs * ys;
+    float Dpq[125];
+    for (int i = 1; i < A.n; i++)
+        Dpq[i] = sqrtf((float)(A.dx[i] * A.dx[i]) + (float)(A.dy[i] * A.dy[i]) + (float)(A.dz[i] * A.dz[i]));
+    int Imax = INT_MIN, Imx = INT_MIN;
+    for (int p = 0; p < n; p++) { if (label[p] > Imax) Imax = label[p]; if (I[p] > Imx) Imx = I[p]; }
+    int Wmax = (int)((double)Imx * (1.0 + (double)fabsf(per) / 100.0));
+    float *f1 = (float *)malloc(sizeof(float) * (size_t)n), *f2 = (float *)malloc(sizeof(float) * (size_t)n);
+    int *mnodes = (int *)malloc(sizeof(int) * ((size_t)n + 1));
+    char *mask = (char *)calloc((size_t)n, 1);
+    if (!f1 || !f2 || !mnodes || !mask) { free(f1); free(f2); free(mnodes); free(mask); return ORC_ENOMEM; }
+    mnodes[0] = 0;
+    for (int p = 0; p < n; p++) {
+        f1[p] = (float)label[p] / (float)Imax;
+        int px = (p % xy) % xs, py = (p % xy) / xs, pz = p / xy;
+        for (int i = 1; i < A.n; i++) {
+            int qx = px + A.dx[i], qy = py + A.dy[i], qz = pz + A.dz[i];
+            if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) continue;
+            int q = qx + qy * xs + qz * xy;
+            if (label[p] != label[q]) { mask[p] = 1; mnodes[++mnodes[0]] = p; break; }
+        }
+    }
+    int ninic = 1;
+    while (ntimes > 0) {
+        memcpy(f2, f1, sizeof(float) * (size_t)n);
+        for (int j = 1; j <= mnodes[0]; j++) {
+            int p = mnodes[j];
+            int px = (p % xy) % xs, py = (p % xy) / xs, pz = p / xy;
+            f2[p] = 0.0f;
+            float sw = 0.0f;
+            for (int i = 1; i < A.n; i++) {
+                int qx = px + A.dx[i], qy = py + A.dy[i], qz = pz + A.dz[i];
+                if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) continue;
+                int q = qx + qy * xs + qz * xy;
+                float w = (float)abs(I[p] - I[q]);
+                float per_pq = (f1[q] < 0.5f) ? per : -per;
+                if (I[p] > I[q]) w = (float)((double)w * (1.0 + (double)per_pq / 100.0));
+                else if (I[p] < I[q]) w = (float)((double)w * (1.0 - (double)per_pq / 100.0));
+                w = (float)Wmax - w;
+                w = w * w; w = w * w; w = w * w;
+                w /= Dpq[i];
+                sw += w;
+                f2[p] += w * f1[q];
+            }
+            f2[p] /= sw;
+        }
+        float *t = f1; f1 = f2; f2 = t;
+        if (S) for (int i = 1; i <= S[0]; i++) { int p = S[i]; f1[p] = (float)label[p] / (float)Imax; }
+        ntimes--;
+        if (ntimes > 0) {
+            int nlast = mnodes[0];
+            for (int j = ninic; j <= mnodes[0]; j++) {
+                int p = mnodes[j];
+                int px = (p % xy) % xs, py = (p % xy) / xs, pz = p / xy;
+                for (int i = 1; i < A.n; i++) {
+                    int qx = px + A.dx[i], qy = py + A.dy[i], qz = pz + A.dz[i];
+                    if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) continue;
+                    int q = qx + qy * xs + qz * xy;
+                    if (!mask[q]) { mask[q] = 1; mnodes[++nlast] = q; }
+                }
+            }
+            ninic = mnodes[0] + 1;
+            mnodes[0] = nlast;
+        }
+    }
+    for (int p = 0; p < n; p++) label[p] = (f1[p] < 0.5f) ? 0 : 1;
+    if (memb) memcpy(memb, f1, sizeof(float) * (size_t)n);
+    free(f1); free(f2); free(mnodes); free(mask);
+    return ORC_OK;
+}

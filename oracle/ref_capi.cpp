@@ -1,0 +1,106 @@
+// ref_capi.cpp -- thin extern "C" wrapper (written for this project) around the
+// reference gft library compiled from /root/reference/gft/src by oracle/Makefile.
+//
+// TEST INFRASTRUCTURE ONLY: it lets tests/ and the cpu_baseline leg of bench.py
+// call the reference's own gft::ift::OIFT_Multi[_PerClass] and
+// ORelax_1_Multi[_PerClass] (gft_ift.h:108-122, 451-465) on in-memory volumes.
+// The built library lives in oracle/_ref/ (git-ignored).
+#include <cstring>
+#include <cstdlib>
+#include <chrono>
+#include "gft.h"
+
+namespace {
+
+gft::sScene32 *make_scene(const int *data, int xs, int ys, int zs)
+{
+    gft::sScene32 *s = gft::Scene32::Create(xs, ys, zs);
+    std::memcpy(s->data, data, sizeof(int) * (size_t)s->n);
+    return s;
+}
+
+int *make_S(const int *S)
+{
+    int *c = (int *)std::calloc((size_t)S[0] + 1, sizeof(int));
+    std::memcpy(c, S, sizeof(int) * ((size_t)S[0] + 1));
+    return c;
+}
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+} // namespace
+
+#define REF_API __attribute__((visibility("default")))
+
+extern "C" {
+
+// OIFT only.  label is in/out (NIL=-1 unlabelled).
+REF_API int ref_oift(const int *I, int xs, int ys, int zs, float adj_r,
+             int use_per_class, float per, const float *per_class, int max_label,
+             const int *S, int *label)
+{
+    gft::sAdjRel3 *A = gft::AdjRel3::Spheric(adj_r);
+    gft::sScene32 *scn = make_scene(I, xs, ys, zs);
+    gft::sScene32 *lab = make_scene(label, xs, ys, zs);
+    int *Sc = S ? make_S(S) : nullptr;
+    if (use_per_class)
+        gft::ift::OIFT_Multi_PerClass(A, scn, per_class, max_label, Sc, lab);
+    else
+        gft::ift::OIFT_Multi(A, scn, per, Sc, lab);
+    std::memcpy(label, lab->data, sizeof(int) * (size_t)lab->n);
+    std::free(Sc);
+    gft::Scene32::Destroy(&lab);
+    gft::Scene32::Destroy(&scn);
+    gft::AdjRel3::Destroy(&A);
+    return 0;
+}
+
+// Relaxation only: label in = OIFT labels, out = relaxed labels.
+REF_API int ref_relax(const int *I, int xs, int ys, int zs, float adj_r,
+              int use_per_class, float per, const float *per_class, int max_label,
+              const int *S, int *label, int ntimes)
+{
+    gft::sAdjRel3 *A = gft::AdjRel3::Spheric(adj_r);
+    gft::sScene32 *scn = make_scene(I, xs, ys, zs);
+    gft::sScene32 *lab = make_scene(label, xs, ys, zs);
+    int *Sc = S ? make_S(S) : nullptr;
+    if (use_per_class)
+        gft::ift::ORelax_1_Multi_PerClass(A, scn, per_class, max_label, Sc, lab, ntimes);
+    else
+        gft::ift::ORelax_1_Multi(A, scn, per, Sc, lab, ntimes);
+    std::memcpy(label, lab->data, sizeof(int) * (size_t)lab->n);
+    std::free(Sc);
+    gft::Scene32::Destroy(&lab);
+    gft::Scene32::Destroy(&scn);
+    gft::AdjRel3::Destroy(&A);
+    return 0;
+}
+
+// OIFT_Multi followed by ORelax_1_Multi on the same scenes, as oiftrelax.cpp:379-401
+// runs them; t_oift / t_relax receive steady_clock seconds around each call.
+REF_API int ref_pipeline(const int *I, int xs, int ys, int zs, float adj_r, float per,
+                 const int *S, int *label, int ntimes, double *t_oift, double *t_relax)
+{
+    gft::sAdjRel3 *A = gft::AdjRel3::Spheric(adj_r);
+    gft::sScene32 *scn = make_scene(I, xs, ys, zs);
+    gft::sScene32 *lab = make_scene(label, xs, ys, zs);
+    int *Sc = make_S(S);
+    double t0 = now_s();
+    gft::ift::OIFT_Multi(A, scn, per, Sc, lab);
+    double t1 = now_s();
+    gft::ift::ORelax_1_Multi(A, scn, per, Sc, lab, ntimes);
+    double t2 = now_s();
+    if (t_oift) *t_oift = t1 - t0;
+    if (t_relax) *t_relax = t2 - t1;
+    std::memcpy(label, lab->data, sizeof(int) * (size_t)lab->n);
+    std::free(Sc);
+    gft::Scene32::Destroy(&lab);
+    gft::Scene32::Destroy(&scn);
+    gft::AdjRel3::Destroy(&A);
+    return 0;
+}
+
+} // extern "C"
