@@ -1,0 +1,130 @@
+/*
+ * roift.h -- C-ABI of the MI355X (gfx950) OIFT f_max + fuzzy relaxation library
+ * (libroift.so).  Plain pointers and sizes only; every call returns a status.
+ *
+ * Conventions (same as the gft library the reference builds on):
+ *   - volumes are x-fastest: voxel p = x + y*xsize + z*xsize*ysize
+ *     (gft_scene32.h:173-179);
+ *   - labels: -1 (NIL) unlabelled, 0 background, 1..L objects;
+ *   - seeds: the gft S array without its count word, i.e. S[1..S[0]] in
+ *     insertion order (gft_ift.cpp:938-943); the order is the FIFO tie order;
+ *   - adjacency: gft::AdjRel3::Spheric(r) (gft_adjrel3.cpp:47-85); r = 1.0
+ *     gives 6-adjacency (a 2-D image is zsize = 1), r = 1.75 gives 26;
+ *   - polarity "per" is alpha x 100 in [-100, 100]; per_class[c] likewise.
+ *
+ * Pointers named d_* are device (HIP) pointers on the context's device; h_*
+ * are host pointers.  Calls on one context are serialised on its stream and
+ * return after the device work is complete.  A context is not thread-safe.
+ */
+#ifndef ROIFT_H
+#define ROIFT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ROIFT_API __attribute__((visibility("default")))
+
+enum roift_status {
+    ROIFT_OK = 0,
+    ROIFT_EINVAL = -1,      /* bad argument (shape, seed index, NIL seed label, duplicate seed, negative image) */
+    ROIFT_ENOMEM = -2,      /* device or host allocation failed */
+    ROIFT_EHIP = -3,        /* a HIP runtime call failed */
+    ROIFT_EUNSUPPORTED = -4 /* valid input outside what this build implements (see roift_last_error) */
+};
+
+enum roift_dtype { ROIFT_I16 = 1, ROIFT_I32 = 2, ROIFT_U16 = 3, ROIFT_U32 = 4, ROIFT_F32 = 5 };
+
+typedef struct roift_ctx roift_ctx;
+
+/* Problem description shared by the entry points. */
+typedef struct {
+    int xsize, ysize, zsize;
+    float adj_radius;          /* Spheric radius: 1.0 (6-adj) or 1.75 (26-adj) */
+    int use_per_class;         /* 0: OIFT_Multi / ORelax_1_Multi polarity from "per"
+                                  (label 0 -> -per, others +per);
+                                  1: *_PerClass polarity per_class[label] */
+    float per;                 /* polarity x100 when use_per_class == 0 */
+    const float *h_per_class;  /* length max_label+1 when use_per_class == 1 */
+    int max_label;
+} roift_problem;
+
+/* ---- context ---------------------------------------------------------------- */
+ROIFT_API int roift_create(roift_ctx **ctx, int device);
+ROIFT_API void roift_destroy(roift_ctx *ctx);
+ROIFT_API const char *roift_last_error(const roift_ctx *ctx);
+/* Run on this hipStream_t (default: a stream the context creates). */
+ROIFT_API int roift_set_stream(roift_ctx *ctx, void *hip_stream);
+/* Stage timings (device events) of the last call, in milliseconds:
+   [0] weights, [1] oift, [2] relax, [3] labels; [4] OIFT level steps, [5] flood rounds. */
+ROIFT_API int roift_last_timings(const roift_ctx *ctx, double *out6);
+
+/* ---- oriented arc-weight builder --------------------------------------------- */
+/* Replaces the weight computed inline in OIFT_Multi (gft_ift.cpp:965-973).
+ * For every forward neighbour slot s (adjacency index i = (n-1)/2 + 1 + s,
+ * q = p + d_i, E = (n-1)/2 slots) writes
+ *     d_wa[s*N + p] = w_obj(p->q),  d_wb[s*N + p] = w_obj(q->p)
+ * with w_obj the +per arc weight; the background (-per) weight is the
+ * transpose: w_bg(p->q) = w_obj(q->p).  Out-of-volume slots hold the dtype's
+ * maximum.  weight_dtype is ROIFT_U16 (requires Wmax <= 65534) or ROIFT_U32.
+ * h_maxI / h_minI (nullable) receive the image extrema.                        */
+ROIFT_API int roift_weights_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                void *d_wa, void *d_wb, int weight_dtype, int32_t *h_maxI, int32_t *h_minI);
+
+/* ---- OIFT under f_max ------------------------------------------------------------ */
+/* Drop-in for gft::ift::OIFT_Multi / OIFT_Multi_PerClass (gft_ift.h:108-122,
+ * gft_ift.cpp:911-1083).  d_label is in/out exactly as the reference's label
+ * scene; the result is bit-identical to the reference's (FIFO tie-breaking
+ * included).  d_cost (nullable) receives the f_max cost map
+ * C(q) = max(C(pred q), w(pred q -> q)) (INT32_MAX for voxels never reached).  */
+ROIFT_API int roift_oift_fmax_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                  const int32_t *d_seeds, int n_seeds, int32_t *d_label, int32_t *d_cost);
+
+/* Same forest on caller-supplied oriented edge weights (layout as
+ * roift_weights_dev writes it, ROIFT_U16 or ROIFT_U32).  Labels for which
+ * h_pol_sign[label] > 0 (label in [0, max_label]) use w_obj, the others w_bg;
+ * with h_pol_sign == NULL label 0 uses w_bg and every other label w_obj.
+ * Used for tie-free (rank) weights, which exceed the image-derived range.     */
+ROIFT_API int roift_oift_edges_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_wa, const void *d_wb,
+                                   int weight_dtype, const int8_t *h_pol_sign, const int32_t *d_seeds, int n_seeds,
+                                   int32_t *d_label, int32_t *d_cost);
+
+/* ---- fuzzy relaxation -------------------------------------------------------------- */
+/* Drop-in for gft::ift::ORelax_1_Multi / ORelax_1_Multi_PerClass
+ * (gft_ift.h:451-465, gft_ift.cpp:8655-9187): d_label in = OIFT labels,
+ * out = argmax labels.  FP32 arithmetic.  d_memb (nullable) receives the final
+ * memberships as planar [nclasses][N] floats, classes in increasing label
+ * order; h_nclasses (nullable) receives the class count.                      */
+ROIFT_API int roift_relax_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                              const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter, float *d_memb,
+                              int32_t *h_nclasses);
+
+/* ---- the whole hot path ------------------------------------------------------------- */
+/* OIFT_Multi followed by ORelax_1_Multi on the same inputs, as the reference
+ * CLI runs them (oiftrelax.cpp:379-401): weights + OIFT + relax + labels.     */
+ROIFT_API int roift_pipeline_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                 const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter,
+                                 int32_t *d_cost, float *d_memb);
+
+/* Host-pointer forms (copy in, run, copy out).  h_image is int32 as gft's Scene32. */
+ROIFT_API int roift_oift_fmax(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image,
+                              const int32_t *h_seeds, int n_seeds, int32_t *h_label, int32_t *h_cost);
+ROIFT_API int roift_relax(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image,
+                          const int32_t *h_seeds, int n_seeds, int32_t *h_label, int niter, float *h_memb);
+
+/* ---- legacy reference C-ABI ---------------------------------------------------------- */
+/* Same symbol and signature as the reference's GPU entry point
+ * (gpu/oift_fmax_gpu.cu:123-147, declared by its caller at
+ * gpu/oiftrelax_fmax_gpu.cpp:21-25).  Computes OIFT_Multi_PerClass with the
+ * given per-class polarities, exactly; h_labels is in/out.  Returns the
+ * number of level steps, or -1 on error.                                        */
+ROIFT_API int oift_fmax_gpu_run(const int *h_image, int *h_labels, int n, int xsize, int ysize, int zsize,
+                                const float *h_per_class, int max_label, const int *h_seeds, int n_seeds);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ROIFT_H */

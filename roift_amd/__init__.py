@@ -1,1 +1,224 @@
-"""roift_amd: MI355X-native OIFT (f_max) + fuzzy relaxation."""
+"""roift_amd -- MI355X (gfx950) OIFT under f_max + fuzzy relaxation.
+
+Python host side of libroift.so (include/roift.h).  `Context` drives the
+C-ABI on torch device tensors (data resident in HBM) or on numpy host arrays;
+`roift_amd.gft` mirrors the reference's gft API (Scene32, AdjRel3.Spheric,
+ift.OIFT_Multi, ift.ORelax_1_Multi, ...) on top of it.
+
+The library must be built (`python -c "import __graft_entry__ as g; g.build()"`
+or `make -C roift_amd/csrc`); nothing here falls back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libroift.so")
+
+OK, EINVAL, ENOMEM, EHIP, EUNSUPPORTED = 0, -1, -2, -3, -4
+I16, I32, U16, U32, F32 = 1, 2, 3, 4, 5
+NIL = -1
+
+EXPORTS = (
+    "roift_create", "roift_destroy", "roift_last_error", "roift_set_stream", "roift_last_timings",
+    "roift_weights_dev", "roift_oift_fmax_dev", "roift_oift_edges_dev", "roift_relax_dev",
+    "roift_pipeline_dev", "roift_oift_fmax", "roift_relax", "oift_fmax_gpu_run",
+)
+
+
+class RoiftError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"roift error {code}: {msg}")
+        self.code = code
+
+
+class Problem(C.Structure):
+    _fields_ = [("xsize", C.c_int), ("ysize", C.c_int), ("zsize", C.c_int), ("adj_radius", C.c_float),
+                ("use_per_class", C.c_int), ("per", C.c_float), ("h_per_class", C.POINTER(C.c_float)),
+                ("max_label", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libroift.so (raises if it was not built -- there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with make -C roift_amd/csrc")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, i32p, i8p, dp = C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int8), C.POINTER(C.c_double)
+        pp = C.POINTER(Problem)
+        L.roift_create.argtypes = [C.POINTER(vp), ip]
+        L.roift_destroy.argtypes = [vp]
+        L.roift_destroy.restype = None
+        L.roift_last_error.argtypes = [vp]
+        L.roift_last_error.restype = C.c_char_p
+        L.roift_set_stream.argtypes = [vp, vp]
+        L.roift_last_timings.argtypes = [vp, dp]
+        L.roift_weights_dev.argtypes = [vp, pp, vp, ip, vp, vp, ip, i32p, i32p]
+        L.roift_oift_fmax_dev.argtypes = [vp, pp, vp, ip, vp, ip, vp, vp]
+        L.roift_oift_edges_dev.argtypes = [vp, pp, vp, vp, ip, i8p, vp, ip, vp, vp]
+        L.roift_relax_dev.argtypes = [vp, pp, vp, ip, vp, ip, vp, ip, vp, i32p]
+        L.roift_pipeline_dev.argtypes = [vp, pp, vp, ip, vp, ip, vp, ip, vp, vp]
+        L.roift_oift_fmax.argtypes = [vp, pp, vp, vp, ip, vp, vp]
+        L.roift_relax.argtypes = [vp, pp, vp, vp, ip, vp, ip, vp]
+        L.oift_fmax_gpu_run.argtypes = [vp, vp, ip, ip, ip, ip, vp, ip, vp, ip]
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    L = lib()
+    return {name: hasattr(L, name) for name in EXPORTS}
+
+
+def make_problem(shape, adj_r=1.0, per=0.0, per_class=None):
+    """shape = (zs, ys, xs) of a C-order volume."""
+    zs, ys, xs = (int(v) for v in shape)
+    pb = Problem(xs, ys, zs, float(adj_r), 0, float(per), None, -1)
+    keep = None
+    if per_class is not None:
+        keep = np.ascontiguousarray(per_class, np.float32)
+        pb.use_per_class = 1
+        pb.h_per_class = keep.ctypes.data_as(C.POINTER(C.c_float))
+        pb.max_label = keep.size - 1
+    return pb, keep
+
+
+def _dtype_code(t):
+    import torch
+    return {torch.int16: I16, torch.int32: I32, torch.uint16: U16 if hasattr(torch, "uint16") else -9,
+            torch.float32: F32}.get(t.dtype, None)
+
+
+class Context:
+    """One roift_ctx on one HIP device; not thread-safe (as the C-ABI)."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.roift_create(C.byref(h), int(device))
+        if rc != OK:
+            raise RoiftError(rc, "roift_create failed")
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.roift_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            raise RoiftError(rc, self._L.roift_last_error(self._h).decode())
+
+    def timings(self):
+        out = (C.c_double * 6)()
+        self._check(self._L.roift_last_timings(self._h, out))
+        keys = ("weights_ms", "oift_ms", "relax_ms", "labels_ms", "oift_steps", "flood_rounds")
+        return dict(zip(keys, list(out)))
+
+    # ---- device (torch) forms -------------------------------------------------------
+    def weights(self, image, adj_r=1.0, per=0.0, wdtype="u16"):
+        """image: torch int16/int32 (zs,ys,xs) on the device -> (wa, wb, maxI, minI)."""
+        import torch
+        pb, _ = make_problem(image.shape, adj_r, per)
+        E = {1.0: 3, 1.75: 13}.get(float(adj_r))
+        wt = torch.int32 if wdtype == "u32" else torch.int16
+        wa = torch.empty((E,) + tuple(image.shape), dtype=wt, device=image.device)
+        wb = torch.empty_like(wa)
+        mx, mn = C.c_int32(), C.c_int32()
+        self._check(self._L.roift_weights_dev(self._h, C.byref(pb), image.data_ptr(), _dtype_code(image),
+                                              wa.data_ptr(), wb.data_ptr(), U32 if wdtype == "u32" else U16,
+                                              C.byref(mx), C.byref(mn)))
+        return wa, wb, mx.value, mn.value
+
+    def oift_fmax(self, image, seeds, label, adj_r=1.0, per=0.0, per_class=None, want_cost=True):
+        """In place on `label` (torch int32 on the device); returns the cost map or None."""
+        import torch
+        assert label.dtype == torch.int32 and label.is_contiguous() and image.is_contiguous()
+        pb, _keep = make_problem(image.shape, adj_r, per, per_class)
+        seeds = seeds.to(device=label.device, dtype=torch.int32).contiguous()
+        cost = torch.empty_like(label) if want_cost else None
+        self._check(self._L.roift_oift_fmax_dev(self._h, C.byref(pb), image.data_ptr(), _dtype_code(image),
+                                                seeds.data_ptr(), int(seeds.numel()), label.data_ptr(),
+                                                cost.data_ptr() if cost is not None else None))
+        return cost
+
+    def oift_edges(self, wa, wb, seeds, label, adj_r=1.0, pol_sign=None, want_cost=True):
+        import torch
+        pb, _ = make_problem(label.shape, adj_r)
+        ps = None
+        if pol_sign is not None:
+            ps = np.ascontiguousarray(pol_sign, np.int8)
+            pb.max_label = ps.size - 1
+        seeds = seeds.to(device=label.device, dtype=torch.int32).contiguous()
+        cost = torch.empty_like(label) if want_cost else None
+        wcode = U16 if wa.element_size() == 2 else U32
+        self._check(self._L.roift_oift_edges_dev(
+            self._h, C.byref(pb), wa.data_ptr(), wb.data_ptr(), wcode,
+            None if ps is None else ps.ctypes.data_as(C.POINTER(C.c_int8)), seeds.data_ptr(), int(seeds.numel()),
+            label.data_ptr(), cost.data_ptr() if cost is not None else None))
+        return cost
+
+    def relax(self, image, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None, want_memb=True):
+        """In place on `label`; returns memberships [nclasses, zs, ys, xs] (float32) or None."""
+        import torch
+        pb, _keep = make_problem(image.shape, adj_r, per, per_class)
+        seeds = seeds.to(device=label.device, dtype=torch.int32).contiguous()
+        memb = torch.empty((8,) + tuple(label.shape), dtype=torch.float32, device=label.device) if want_memb else None
+        nc = C.c_int32(0)
+        self._check(self._L.roift_relax_dev(self._h, C.byref(pb), image.data_ptr(), _dtype_code(image),
+                                            seeds.data_ptr(), int(seeds.numel()), label.data_ptr(), int(niter),
+                                            memb.data_ptr() if memb is not None else None, C.byref(nc)))
+        return memb[: nc.value] if memb is not None else None
+
+    def pipeline(self, image, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None, cost=None, memb=None):
+        """weights + OIFT + relax + labels on device tensors (label in/out)."""
+        pb, _keep = make_problem(image.shape, adj_r, per, per_class)
+        self._check(self._L.roift_pipeline_dev(self._h, C.byref(pb), image.data_ptr(), _dtype_code(image),
+                                               seeds.data_ptr(), int(seeds.numel()), label.data_ptr(), int(niter),
+                                               cost.data_ptr() if cost is not None else None,
+                                               memb.data_ptr() if memb is not None else None))
+
+    # ---- host (numpy) forms ------------------------------------------------------------
+    def oift_fmax_host(self, image, seeds, label, adj_r=1.0, per=0.0, per_class=None, want_cost=True):
+        img = np.ascontiguousarray(image, np.int32)
+        lab = np.ascontiguousarray(label, np.int32).copy()
+        S = np.ascontiguousarray(seeds, np.int32)
+        cost = np.empty_like(lab) if want_cost else None
+        pb, _keep = make_problem(img.shape, adj_r, per, per_class)
+        self._check(self._L.roift_oift_fmax(self._h, C.byref(pb), img.ctypes.data, S.ctypes.data, S.size,
+                                            lab.ctypes.data, None if cost is None else cost.ctypes.data))
+        return lab, cost
+
+    def relax_host(self, image, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None):
+        img = np.ascontiguousarray(image, np.int32)
+        lab = np.ascontiguousarray(label, np.int32).copy()
+        S = np.ascontiguousarray(seeds, np.int32)
+        memb = np.empty((8,) + img.shape, np.float32)
+        pb, _keep = make_problem(img.shape, adj_r, per, per_class)
+        self._check(self._L.roift_relax(self._h, C.byref(pb), img.ctypes.data, S.ctypes.data, S.size,
+                                        lab.ctypes.data, int(niter), memb.ctypes.data))
+        return lab, memb
+
+
+def oift_fmax_gpu_run(h_image, h_labels, xsize, ysize, zsize, per_class, seeds):
+    """The reference's legacy C entry point (gpu/oift_fmax_gpu.cu:123-147), h_labels in/out."""
+    img = np.ascontiguousarray(h_image, np.int32)
+    assert h_labels.dtype == np.int32 and h_labels.flags.c_contiguous
+    pc = np.ascontiguousarray(per_class, np.float32)
+    S = np.ascontiguousarray(seeds, np.int32)
+    return lib().oift_fmax_gpu_run(img.ctypes.data, h_labels.ctypes.data, img.size, xsize, ysize, zsize,
+                                   pc.ctypes.data, pc.size - 1, S.ctypes.data, S.size)

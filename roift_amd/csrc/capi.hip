@@ -1,0 +1,361 @@
+// capi.hip -- the extern "C" boundary of libroift.so (declared in include/roift.h).
+#include <cmath>
+#include <set>
+
+#include "ctx.hpp"
+
+using namespace roift;
+
+namespace {
+
+// Spheric(r) neighbourhood size including the centre (gft_adjrel3.cpp:47-85).
+int spheric_n(float r)
+{
+    const int r0 = (int)r, r2 = (int)(r * r);
+    int n = 0;
+    for (int z = -r0; z <= r0; z++)
+        for (int y = -r0; y <= r0; y++)
+            for (int x = -r0; x <= r0; x++)
+                if (x * x + y * y + z * z <= r2) n++;
+    return n;
+}
+
+Dims dims_of(const roift_problem *pb)
+{
+    ROIFT_REQUIRE(pb != nullptr, ROIFT_EINVAL, "null problem");
+    ROIFT_REQUIRE(pb->xsize > 0 && pb->ysize > 0 && pb->zsize > 0, ROIFT_EINVAL, "non-positive volume size");
+    Dims d{pb->xsize, pb->ysize, pb->zsize, (long long)pb->xsize * pb->ysize * pb->zsize};
+    ROIFT_REQUIRE(d.n < (1ll << 31), ROIFT_EINVAL, "volume larger than 2^31-1 voxels");
+    return d;
+}
+
+int nadj_of(const roift_problem *pb)
+{
+    const int n = spheric_n(pb->adj_radius);
+    ROIFT_REQUIRE(n == 7 || n == 27, ROIFT_EUNSUPPORTED, "only Spheric(1.0) (6-adj) and Spheric(1.75) (26-adj)");
+    return n;
+}
+
+// Split the labels into the two weight classes of the edge arrays built with
+// +P: class 1 uses w_P (arcs as stored), class 0 uses w_{-P} (transposed).
+struct Polarity {
+    float P = 0.f;
+    LabelClasses lc;
+};
+
+Polarity polarity_of(const roift_problem *pb)
+{
+    Polarity po;
+    if (!pb->use_per_class) {
+        ROIFT_REQUIRE(std::fabs(pb->per) <= 100.0f, ROIFT_EINVAL, "|per| > 100");
+        po.P = pb->per;
+        po.lc.cls = {0};     // label 0 -> -per (gft_ift.cpp:968)
+        po.lc.default_cls = 1;
+        return po;
+    }
+    ROIFT_REQUIRE(pb->h_per_class != nullptr && pb->max_label >= 0, ROIFT_EINVAL, "per_class missing");
+    std::set<float> vals;
+    for (int c = 0; c <= pb->max_label; c++) {
+        ROIFT_REQUIRE(std::fabs(pb->h_per_class[c]) <= 100.0f, ROIFT_EINVAL, "|per_class| > 100");
+        vals.insert(pb->h_per_class[c]);
+    }
+    float P = *vals.rbegin();
+    if (vals.size() == 2) {
+        ROIFT_REQUIRE(*vals.begin() == -P, ROIFT_EUNSUPPORTED,
+                      "per_class with two values that are not +-P is not implemented");
+    } else {
+        ROIFT_REQUIRE(vals.size() == 1, ROIFT_EUNSUPPORTED, "per_class with more than two polarities is not implemented");
+        if (P < 0) P = -P;   // a single value -P: every label uses the transposed arcs of +P
+    }
+    po.P = P;
+    po.lc.cls.resize(pb->max_label + 1);
+    for (int c = 0; c <= pb->max_label; c++) po.lc.cls[c] = (pb->h_per_class[c] == P) ? 1 : 0;
+    // labels outside [0, max_label] get polarity 0 (gft_ift.cpp:1058-1062)
+    po.lc.default_cls = (P == 0.0f) ? 1 : -1;
+    return po;
+}
+
+RelaxSpec relax_spec_of(const roift_problem *pb)
+{
+    RelaxSpec rs;
+    rs.use_per_class = pb->use_per_class;
+    rs.per = pb->per;
+    rs.max_label = pb->max_label;
+    if (pb->use_per_class) {
+        ROIFT_REQUIRE(pb->h_per_class != nullptr && pb->max_label >= 0, ROIFT_EINVAL, "per_class missing");
+        rs.per_class.assign(pb->h_per_class, pb->h_per_class + pb->max_label + 1);
+    }
+    return rs;
+}
+
+template <typename F>
+int guarded(roift_ctx *ctx, F &&f)
+{
+    if (!ctx) return ROIFT_EINVAL;
+    try {
+        ROIFT_HIP_CHECK(hipSetDevice(ctx->device));
+        ctx->last_error.clear();
+        return f();
+    } catch (const Error &e) {
+        ctx->last_error = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        ctx->last_error = e.what();
+        return ROIFT_ENOMEM;
+    }
+}
+
+struct Weights {
+    void *wa, *wb;
+    int dtype;
+    WeightInfo info;
+};
+
+Weights weights_into_ctx(roift_ctx &c, const Dims &d, int nadj, const void *img, int idt, float P)
+{
+    const long long slots = (long long)((nadj - 1) / 2) * d.n;
+    Weights w{};
+    w.dtype = ROIFT_U16;
+    w.wa = buf<uint16_t>(c, "pipe.wa", slots);
+    w.wb = buf<uint16_t>(c, "pipe.wb", slots);
+    try {
+        w.info = build_weights(c, d, nadj, img, idt, P, w.wa, w.wb, ROIFT_U16);
+    } catch (const Error &e) {
+        if (e.code != ROIFT_EUNSUPPORTED) throw;
+        w.dtype = ROIFT_U32;
+        w.wa = buf<uint32_t>(c, "pipe.wa32", slots);
+        w.wb = buf<uint32_t>(c, "pipe.wb32", slots);
+        w.info = build_weights(c, d, nadj, img, idt, P, w.wa, w.wb, ROIFT_U32);
+    }
+    ROIFT_REQUIRE(w.info.minI >= 0, ROIFT_EINVAL, "image has negative values (shift it to >= 0 first)");
+    return w;
+}
+
+void ev_record(roift_ctx &c, int i) { ROIFT_HIP_CHECK(hipEventRecord(c.ev[i], c.stream)); }
+
+double ev_ms(roift_ctx &c, int a, int b)
+{
+    float ms = 0.f;
+    ROIFT_HIP_CHECK(hipEventSynchronize(c.ev[b]));
+    ROIFT_HIP_CHECK(hipEventElapsedTime(&ms, c.ev[a], c.ev[b]));
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+ROIFT_API int roift_create(roift_ctx **out, int device)
+{
+    if (!out) return ROIFT_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return ROIFT_EHIP;
+    auto *c = new roift_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_ctr, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+        delete c;
+        return ROIFT_EHIP;
+    }
+    c->owns_stream = true;
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) return ROIFT_EHIP;
+    *out = c;
+    return ROIFT_OK;
+}
+
+ROIFT_API void roift_destroy(roift_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &kv : c->bufs) (void)hipFree(kv.second.first);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->h_ctr) (void)hipHostFree(c->h_ctr);
+    if (c->owns_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+ROIFT_API const char *roift_last_error(const roift_ctx *c) { return c ? c->last_error.c_str() : "null context"; }
+
+ROIFT_API int roift_set_stream(roift_ctx *c, void *stream)
+{
+    return guarded(c, [&]() -> int {
+        if (c->owns_stream && c->stream) ROIFT_HIP_CHECK(hipStreamDestroy(c->stream));
+        c->stream = (hipStream_t)stream;
+        c->owns_stream = false;
+        return ROIFT_OK;
+    });
+}
+
+ROIFT_API int roift_last_timings(const roift_ctx *c, double *out6)
+{
+    if (!c || !out6) return ROIFT_EINVAL;
+    for (int i = 0; i < 6; i++) out6[i] = c->timings[i];
+    return ROIFT_OK;
+}
+
+ROIFT_API int roift_weights_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                void *d_wa, void *d_wb, int weight_dtype, int32_t *h_maxI, int32_t *h_minI)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        const int nadj = nadj_of(pb);
+        ROIFT_REQUIRE(!pb->use_per_class, ROIFT_EINVAL, "the weight builder takes a single polarity (per)");
+        const WeightInfo wi = build_weights(*ctx, d, nadj, d_image, image_dtype, pb->per, d_wa, d_wb, weight_dtype);
+        if (h_maxI) *h_maxI = wi.maxI;
+        if (h_minI) *h_minI = wi.minI;
+        return ROIFT_OK;
+    });
+}
+
+ROIFT_API int roift_oift_fmax_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                  const int32_t *d_seeds, int n_seeds, int32_t *d_label, int32_t *d_cost)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        const int nadj = nadj_of(pb);
+        const Polarity po = polarity_of(pb);
+        ev_record(*ctx, 0);
+        const Weights w = weights_into_ctx(*ctx, d, nadj, d_image, image_dtype, po.P);
+        ev_record(*ctx, 1);
+        const int lv = run_oift(*ctx, d, nadj, w.wa, w.wb, w.dtype, po.lc, d_seeds, n_seeds, d_label, d_cost);
+        ev_record(*ctx, 2);
+        ctx->timings[0] = ev_ms(*ctx, 0, 1);
+        ctx->timings[1] = ev_ms(*ctx, 1, 2);
+        return lv >= 0 ? ROIFT_OK : lv;
+    });
+}
+
+ROIFT_API int roift_oift_edges_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_wa, const void *d_wb,
+                                   int weight_dtype, const int8_t *h_pol_sign, const int32_t *d_seeds, int n_seeds,
+                                   int32_t *d_label, int32_t *d_cost)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        const int nadj = nadj_of(pb);
+        LabelClasses lc;
+        if (h_pol_sign) {
+            ROIFT_REQUIRE(pb->max_label >= 0, ROIFT_EINVAL, "pol_sign needs max_label");
+            lc.cls.resize(pb->max_label + 1);
+            for (int c = 0; c <= pb->max_label; c++) lc.cls[c] = h_pol_sign[c] > 0 ? 1 : 0;
+            lc.default_cls = 1;
+        } else {
+            lc.cls = {0};
+            lc.default_cls = 1;
+        }
+        ev_record(*ctx, 1);
+        const int lv = run_oift(*ctx, d, nadj, d_wa, d_wb, weight_dtype, lc, d_seeds, n_seeds, d_label, d_cost);
+        ev_record(*ctx, 2);
+        ctx->timings[1] = ev_ms(*ctx, 1, 2);
+        return lv >= 0 ? ROIFT_OK : lv;
+    });
+}
+
+ROIFT_API int roift_relax_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                              const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter, float *d_memb,
+                              int32_t *h_nclasses)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        const int nadj = nadj_of(pb);
+        const RelaxSpec rs = relax_spec_of(pb);
+        ev_record(*ctx, 2);
+        const WeightInfo wi = image_extrema(*ctx, d, d_image, image_dtype);
+        run_relax(*ctx, d, nadj, d_image, image_dtype, rs, d_seeds, n_seeds, d_label, niter, d_memb, h_nclasses, wi.maxI);
+        ev_record(*ctx, 3);
+        ctx->timings[2] = ev_ms(*ctx, 2, 3);
+        return ROIFT_OK;
+    });
+}
+
+ROIFT_API int roift_pipeline_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                 const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter, int32_t *d_cost,
+                                 float *d_memb)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        const int nadj = nadj_of(pb);
+        const Polarity po = polarity_of(pb);
+        const RelaxSpec rs = relax_spec_of(pb);
+        ev_record(*ctx, 0);
+        const Weights w = weights_into_ctx(*ctx, d, nadj, d_image, image_dtype, po.P);
+        ev_record(*ctx, 1);
+        run_oift(*ctx, d, nadj, w.wa, w.wb, w.dtype, po.lc, d_seeds, n_seeds, d_label, d_cost);
+        ev_record(*ctx, 2);
+        run_relax(*ctx, d, nadj, d_image, image_dtype, rs, d_seeds, n_seeds, d_label, niter, d_memb, nullptr, w.info.maxI);
+        ev_record(*ctx, 3);
+        ctx->timings[0] = ev_ms(*ctx, 0, 1);
+        ctx->timings[1] = ev_ms(*ctx, 1, 2);
+        ctx->timings[2] = ev_ms(*ctx, 2, 3);
+        return ROIFT_OK;
+    });
+}
+
+ROIFT_API int roift_oift_fmax(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image, const int32_t *h_seeds,
+                              int n_seeds, int32_t *h_label, int32_t *h_cost)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        int32_t *img = buf<int32_t>(*ctx, "host.img", d.n);
+        int32_t *lab = buf<int32_t>(*ctx, "host.lab", d.n);
+        int32_t *cost = h_cost ? buf<int32_t>(*ctx, "host.cost", d.n) : nullptr;
+        int32_t *S = buf<int32_t>(*ctx, "host.seeds", n_seeds > 0 ? n_seeds : 1);
+        ROIFT_HIP_CHECK(hipMemcpyAsync(img, h_image, 4 * d.n, hipMemcpyHostToDevice, ctx->stream));
+        ROIFT_HIP_CHECK(hipMemcpyAsync(lab, h_label, 4 * d.n, hipMemcpyHostToDevice, ctx->stream));
+        if (n_seeds > 0)
+            ROIFT_HIP_CHECK(hipMemcpyAsync(S, h_seeds, 4 * (size_t)n_seeds, hipMemcpyHostToDevice, ctx->stream));
+        int rc = roift_oift_fmax_dev(ctx, pb, img, ROIFT_I32, S, n_seeds, lab, cost);
+        if (rc != ROIFT_OK) return rc;
+        ROIFT_HIP_CHECK(hipMemcpyAsync(h_label, lab, 4 * d.n, hipMemcpyDeviceToHost, ctx->stream));
+        if (h_cost) ROIFT_HIP_CHECK(hipMemcpyAsync(h_cost, cost, 4 * d.n, hipMemcpyDeviceToHost, ctx->stream));
+        ROIFT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return ROIFT_OK;
+    });
+}
+
+ROIFT_API int roift_relax(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image, const int32_t *h_seeds,
+                          int n_seeds, int32_t *h_label, int niter, float *h_memb)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        int32_t *img = buf<int32_t>(*ctx, "host.img", d.n);
+        int32_t *lab = buf<int32_t>(*ctx, "host.lab", d.n);
+        int32_t *S = buf<int32_t>(*ctx, "host.seeds", n_seeds > 0 ? n_seeds : 1);
+        ROIFT_HIP_CHECK(hipMemcpyAsync(img, h_image, 4 * d.n, hipMemcpyHostToDevice, ctx->stream));
+        ROIFT_HIP_CHECK(hipMemcpyAsync(lab, h_label, 4 * d.n, hipMemcpyHostToDevice, ctx->stream));
+        if (n_seeds > 0)
+            ROIFT_HIP_CHECK(hipMemcpyAsync(S, h_seeds, 4 * (size_t)n_seeds, hipMemcpyHostToDevice, ctx->stream));
+        int32_t nc = 0;
+        float *memb = nullptr;
+        if (h_memb) memb = buf<float>(*ctx, "host.memb", 8 * d.n);
+        int rc = roift_relax_dev(ctx, pb, img, ROIFT_I32, S, n_seeds, lab, niter, memb, &nc);
+        if (rc != ROIFT_OK) return rc;
+        ROIFT_HIP_CHECK(hipMemcpyAsync(h_label, lab, 4 * d.n, hipMemcpyDeviceToHost, ctx->stream));
+        if (h_memb)
+            ROIFT_HIP_CHECK(hipMemcpyAsync(h_memb, memb, 4 * d.n * (size_t)nc, hipMemcpyDeviceToHost, ctx->stream));
+        ROIFT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return ROIFT_OK;
+    });
+}
+
+ROIFT_API int oift_fmax_gpu_run(const int *h_image, int *h_labels, int n, int xsize, int ysize, int zsize,
+                                const float *h_per_class, int max_label, const int *h_seeds, int n_seeds)
+{
+    if (!h_image || !h_labels || !h_per_class || max_label < 0 || n != xsize * ysize * zsize) return -1;
+    if (n_seeds <= 0) return 0;  // nothing to propagate
+    roift_ctx *ctx = nullptr;
+    if (roift_create(&ctx, 0) != ROIFT_OK) return -1;
+    roift_problem pb{xsize, ysize, zsize, 1.0f, 1, 0.0f, h_per_class, max_label};
+    // the reference maps negative seed labels to background (gpu/oift_fmax_gpu.cu:178-181)
+    for (int i = 0; i < n_seeds; i++)
+        if (h_seeds[i] >= 0 && h_seeds[i] < n && h_labels[h_seeds[i]] < 0) h_labels[h_seeds[i]] = 0;
+    int rc = roift_oift_fmax(ctx, &pb, h_image, h_seeds, n_seeds, h_labels, nullptr);
+    const int steps = (int)ctx->timings[4];
+    roift_destroy(ctx);
+    return rc == ROIFT_OK ? steps : -1;
+}
+
+}  // extern "C"

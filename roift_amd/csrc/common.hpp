@@ -1,0 +1,110 @@
+// common.hpp -- shared device/host definitions for the gfx950 OIFT + relaxation kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/roift.h"
+
+namespace roift {
+
+constexpr uint32_t INF32 = 0xFFFFFFFFu;
+constexpr uint64_t INF64 = 0xFFFFFFFFFFFFFFFFull;
+constexpr int NIL = -1;
+
+// Neighbour displacements in gft::AdjRel3::Spheric order (gft_adjrel3.cpp:47-85):
+// raster dz, dy, dx over the ball with the centre moved to index 0.
+// Spheric(1.0): -z, -y, -x, +x, +y, +z.  Spheric(1.75): the 26 neighbours in
+// raster order.  Index i and n-i are opposite; indices > (n-1)/2 are "forward".
+template <int NADJ> struct Adjacency;
+
+template <> struct Adjacency<7> {
+    static constexpr int n = 7, half = 3;
+    __host__ __device__ static constexpr int dx(int i) { return i == 3 ? -1 : (i == 4 ? 1 : 0); }
+    __host__ __device__ static constexpr int dy(int i) { return i == 2 ? -1 : (i == 5 ? 1 : 0); }
+    __host__ __device__ static constexpr int dz(int i) { return i == 1 ? -1 : (i == 6 ? 1 : 0); }
+};
+
+template <> struct Adjacency<27> {
+    static constexpr int n = 27, half = 13;
+    // raster index r = (dz+1)*9 + (dy+1)*3 + (dx+1); i in 1..13 -> r = i-1, i in 14..26 -> r = i
+    __host__ __device__ static constexpr int raster(int i) { return i <= 13 ? i - 1 : i; }
+    __host__ __device__ static constexpr int dx(int i) { return raster(i) % 3 - 1; }
+    __host__ __device__ static constexpr int dy(int i) { return (raster(i) / 3) % 3 - 1; }
+    __host__ __device__ static constexpr int dz(int i) { return raster(i) / 9 - 1; }
+};
+
+struct Dims {
+    int xs, ys, zs;
+    long long n;
+};
+
+__host__ __device__ inline bool in_volume(const Dims &d, int x, int y, int z)
+{
+    return (unsigned)x < (unsigned)d.xs && (unsigned)y < (unsigned)d.ys && (unsigned)z < (unsigned)d.zs;
+}
+
+// Oriented edge weights: slot s of voxel p stores the edge to q = p + d_(half+1+s):
+//   wa[s][p] = w_obj(p->q), wb[s][p] = w_obj(q->p);  w_bg(p->q) = w_obj(q->p).
+// cls = 1 selects w_obj, cls = 0 selects w_bg.
+template <int NADJ, typename WT>
+__device__ inline uint32_t arc_weight(const WT *__restrict__ wa, const WT *__restrict__ wb, long long n, int i,
+                                      long long p, long long q, int cls)
+{
+    using A = Adjacency<NADJ>;
+    if (i > A::half) {
+        const long long s = i - A::half - 1;
+        return cls ? (uint32_t)wa[s * n + p] : (uint32_t)wb[s * n + p];
+    } else {
+        const long long s = (A::n - i) - A::half - 1;
+        return cls ? (uint32_t)wb[s * n + q] : (uint32_t)wa[s * n + q];
+    }
+}
+
+// Grid-stride helpers.
+__device__ inline long long gtid() { return (long long)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ inline long long gstride() { return (long long)gridDim.x * blockDim.x; }
+
+// Wave-aggregated append to a global list: one atomicAdd per wavefront.
+__device__ inline uint32_t wave_append_index(uint32_t *counter, bool want)
+{
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0) return INF32;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (!want) return INF32;
+    const unsigned long long below = mask & ((1ull << lane) - 1ull);
+    return base + (uint32_t)__popcll(below);
+}
+
+// Fill without hipMemsetAsync (kernels are safe inside captured graphs).
+template <typename T>
+__global__ void k_fill(T *__restrict__ a, long long n, T v)
+{
+    for (long long i = gtid(); i < n; i += gstride()) a[i] = v;
+}
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+}  // namespace roift
+
+#define ROIFT_HIP_CHECK(expr)                                                                   \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess)                                                                   \
+            throw ::roift::Error{ROIFT_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e)}; \
+    } while (0)
+
+#define ROIFT_REQUIRE(cond, code, text)                      \
+    do {                                                     \
+        if (!(cond)) throw ::roift::Error{(code), (text)};   \
+    } while (0)

@@ -1,0 +1,87 @@
+// ctx.hpp -- per-device context: stream, cached device buffers, pinned counters, timings.
+#pragma once
+
+#include <map>
+#include <string>
+
+#include "common.hpp"
+
+struct roift_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool owns_stream = false;
+    std::string last_error;
+    std::map<std::string, std::pair<void *, size_t>> bufs;
+    uint32_t *h_ctr = nullptr;      // pinned host mirror of the device counters
+    double timings[6] = {0, 0, 0, 0, 0, 0};
+    hipEvent_t ev[8] = {};
+};
+
+namespace roift {
+
+// Device buffer owned by the context, grown on demand and reused across calls.
+template <typename T>
+T *buf(roift_ctx &c, const std::string &name, size_t count)
+{
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = 256;
+    auto it = c.bufs.find(name);
+    if (it != c.bufs.end() && it->second.second >= bytes) return static_cast<T *>(it->second.first);
+    if (it != c.bufs.end()) {
+        ROIFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+        ROIFT_HIP_CHECK(hipFree(it->second.first));
+        c.bufs.erase(it);
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) throw Error{ROIFT_ENOMEM, "hipMalloc failed for " + name};
+    c.bufs[name] = {p, bytes};
+    return static_cast<T *>(p);
+}
+
+inline unsigned grid_for(long long n, int block = 256, long long cap = 8192)
+{
+    long long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+template <typename T>
+void fill(roift_ctx &c, T *p, long long n, T v)
+{
+    hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n)), dim3(256), 0, c.stream, p, n, v);
+    ROIFT_HIP_CHECK(hipGetLastError());
+}
+
+// Stage results of the oriented weight builder.
+struct WeightInfo {
+    int32_t maxI = 0, minI = 0;
+};
+
+// Entry points implemented in the .hip translation units.
+WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,
+                         void *d_wa, void *d_wb, int weight_dtype);
+WeightInfo image_extrema(roift_ctx &c, const Dims &d, const void *d_image, int image_dtype);
+
+// Class of each label for the OIFT: 1 = w_obj, 0 = w_bg (transposed).  Labels
+// outside [0, max_label] use default_cls.
+struct LabelClasses {
+    std::vector<int8_t> cls;  // index = label
+    int default_cls = 1;
+};
+
+int run_oift(roift_ctx &c, const Dims &d, int nadj, const void *d_wa, const void *d_wb, int weight_dtype,
+             const LabelClasses &lc, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int32_t *d_cost);
+
+struct RelaxSpec {
+    int use_per_class = 0;
+    float per = 0.f;
+    std::vector<float> per_class;  // indexed by label, length max_label+1
+    int max_label = -1;
+};
+
+int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, const RelaxSpec &rs,
+              const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter, float *d_memb, int32_t *nclasses_out,
+              int32_t maxI);
+
+}  // namespace roift

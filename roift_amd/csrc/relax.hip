@@ -1,0 +1,341 @@
+// relax.hip -- multi-class fuzzy relaxation (ORelax_1_Multi / _PerClass) for gfx950.
+//
+// The reference (gft_ift.cpp:8655-9187) updates only a band that grows one
+// voxel ring per iteration.  Outside the band every neighbour carries the same
+// one-hot vector, so the weighted average sum(w*1)/sum(w) equals 1 exactly and
+// sum(w*0)/sum(w) equals 0: a dense update over the whole volume gives the same
+// numbers and needs no mask, no mask dilation and no per-iteration memcpy.
+// Seeds are reset to one-hot after each iteration, which the kernel does by
+// writing the one-hot vector directly.
+//
+// Two classes (background + one object label) store one FP32 channel, the
+// background membership; the object membership is its complement.  More
+// classes store planar [nclasses][N] FP32 channels.
+//
+// Arithmetic follows the reference's FP32 operation order per neighbour, in
+// Spheric order, with FMA contraction disabled at compile time.
+//
+// Bound: HBM.  Algorithmic bytes per voxel and iteration: image read
+// (2 B int16) + membership read (4 B/channel) + membership write (4 B/channel).
+#include <algorithm>
+
+#include "ctx.hpp"
+
+namespace roift {
+namespace {
+
+constexpr uint8_t NOSEED = 255;
+constexpr int NC_MAX = 8;
+
+struct RelaxArgs {
+    Dims d;
+    float per;          // OIFT_Multi form
+    float pcm[NC_MAX];  // per-class form, mapped to compact classes
+    float Wmax;
+    float Dpq[27];
+    int use_pc;
+    int nc;
+    int bg;
+    int cls_label[NC_MAX];
+};
+
+template <int NADJ, typename IT>
+__global__ __launch_bounds__(256) void k_relax2(const IT *__restrict__ img, RelaxArgs a, const float *__restrict__ fin,
+                                                 float *__restrict__ fout, const uint8_t *__restrict__ seedcls,
+                                                 int32_t *__restrict__ label_out)
+{
+    using A = Adjacency<NADJ>;
+    const Dims d = a.d;
+    const long long xy = (long long)d.xs * d.ys;
+    for (long long p = gtid(); p < d.n; p += gstride()) {
+        const uint8_t sc = seedcls[p];
+        float v;
+        if (sc != NOSEED) {
+            v = (sc == a.bg) ? 1.0f : 0.0f;
+        } else {
+            const int x = (int)(p % d.xs), y = (int)((p / d.xs) % d.ys), z = (int)(p / xy);
+            const int Ip = (int)img[p];
+            float sw = 0.0f, acc = 0.0f;
+#pragma unroll
+            for (int k = 1; k < NADJ; k++) {
+                const int qx = x + A::dx(k), qy = y + A::dy(k), qz = z + A::dz(k);
+                if (!in_volume(d, qx, qy, qz)) continue;
+                const long long q = qx + (long long)qy * d.xs + (long long)qz * xy;
+                const int Iq = (int)img[q];
+                const float fq = fin[q];
+                float w = fabsf((float)(Ip - Iq));
+                // soft polarity from the neighbour's memberships (gft_ift.cpp:8800-8802, 9077-9079)
+                const float s = a.use_pc ? (a.pcm[a.bg] * fq + a.pcm[1 - a.bg] * (1.0f - fq))
+                                         : a.per * (2.0f * fq - 1.0f);
+                if (Ip > Iq) w *= (1.0f + s / 100.0f);
+                else if (Ip < Iq) w *= (1.0f - s / 100.0f);
+                w = a.Wmax - w;
+                if (w < 0.0f) w = 0.0f;
+                w = w * w;
+                w = w * w;
+                w = w * w;
+                if (NADJ != 7) w /= a.Dpq[k];
+                sw += w;
+                acc += w * fq;
+            }
+            v = (sw > 0.0f) ? acc / sw : fin[p];
+        }
+        fout[p] = v;
+        if (label_out) label_out[p] = ((1.0f - v) > v) ? a.cls_label[1 - a.bg] : a.cls_label[a.bg];
+    }
+}
+
+template <int NADJ, typename IT>
+__global__ __launch_bounds__(256) void k_relaxn(const IT *__restrict__ img, RelaxArgs a, const float *__restrict__ fin,
+                                                 float *__restrict__ fout, const uint8_t *__restrict__ seedcls,
+                                                 int32_t *__restrict__ label_out)
+{
+    using A = Adjacency<NADJ>;
+    const Dims d = a.d;
+    const long long xy = (long long)d.xs * d.ys;
+    const int nc = a.nc;
+    for (long long p = gtid(); p < d.n; p += gstride()) {
+        const uint8_t sc = seedcls[p];
+        float v[NC_MAX];
+        if (sc != NOSEED) {
+            for (int c = 0; c < nc; c++) v[c] = (c == sc) ? 1.0f : 0.0f;
+        } else {
+            const int x = (int)(p % d.xs), y = (int)((p / d.xs) % d.ys), z = (int)(p / xy);
+            const int Ip = (int)img[p];
+            float sw = 0.0f;
+            for (int c = 0; c < nc; c++) v[c] = 0.0f;
+            for (int k = 1; k < NADJ; k++) {
+                const int qx = x + A::dx(k), qy = y + A::dy(k), qz = z + A::dz(k);
+                if (!in_volume(d, qx, qy, qz)) continue;
+                const long long q = qx + (long long)qy * d.xs + (long long)qz * xy;
+                const int Iq = (int)img[q];
+                float w = fabsf((float)(Ip - Iq));
+                float s;
+                if (a.use_pc) {
+                    s = 0.0f;
+                    for (int c = 0; c < nc; c++) s += a.pcm[c] * fin[c * d.n + q];
+                } else {
+                    s = a.per * (2.0f * fin[a.bg * d.n + q] - 1.0f);
+                }
+                if (Ip > Iq) w *= (1.0f + s / 100.0f);
+                else if (Ip < Iq) w *= (1.0f - s / 100.0f);
+                w = a.Wmax - w;
+                if (w < 0.0f) w = 0.0f;
+                w = w * w;
+                w = w * w;
+                w = w * w;
+                if (NADJ != 7) w /= a.Dpq[k];
+                sw += w;
+                for (int c = 0; c < nc; c++) v[c] += w * fin[c * d.n + q];
+            }
+            if (sw > 0.0f) {
+                for (int c = 0; c < nc; c++) v[c] /= sw;
+            } else {
+                for (int c = 0; c < nc; c++) v[c] = fin[c * d.n + p];
+            }
+        }
+        int best = 0;
+        for (int c = 0; c < nc; c++) {
+            fout[c * d.n + p] = v[c];
+            if (c > 0 && v[c] > v[best]) best = c;
+        }
+        if (label_out) label_out[p] = a.cls_label[best];
+    }
+}
+
+// Seed labels and their maximum (for the class list), with seed validation.
+__global__ void k_seed_labels(const int32_t *__restrict__ S, int ns, long long n, const int32_t *__restrict__ label,
+                              int32_t *__restrict__ out, int *__restrict__ info)
+{
+    for (long long i = gtid(); i < ns; i += gstride()) {
+        const int v = S[i];
+        int lb = NIL;
+        if (v >= 0 && v < n) lb = label[v];
+        else atomicOr(&info[1], 1);
+        out[i] = lb;
+        atomicMax(&info[0], lb);
+    }
+}
+
+__global__ void k_mark_present(const int32_t *__restrict__ sl, int ns, int maxl, uint8_t *__restrict__ present)
+{
+    for (long long i = gtid(); i < ns; i += gstride()) {
+        const int lb = sl[i];
+        if (lb >= 0 && lb <= maxl) present[lb] = 1;
+    }
+}
+
+// Initial memberships: one-hot of the class of the OIFT label (gft_ift.cpp:8745-8752);
+// seeds' classes for the per-iteration reset (gft_ift.cpp:8839-8855).
+__global__ void k_relax_init(long long n, const int32_t *__restrict__ label, const int16_t *__restrict__ l2c, int maxl,
+                             int nc, int bg, float *__restrict__ f, uint8_t *__restrict__ seedcls)
+{
+    for (long long p = gtid(); p < n; p += gstride()) {
+        const int lb = label[p];
+        int c = bg;
+        if (lb >= 0 && lb <= maxl && l2c[lb] >= 0) c = l2c[lb];
+        if (nc == 2) {
+            f[p] = (c == bg) ? 1.0f : 0.0f;
+        } else {
+            for (int k = 0; k < nc; k++) f[k * n + p] = (k == c) ? 1.0f : 0.0f;
+        }
+        seedcls[p] = NOSEED;
+    }
+}
+
+__global__ void k_seed_classes(const int32_t *__restrict__ S, int ns, const int32_t *__restrict__ label,
+                               const int16_t *__restrict__ l2c, int maxl, int bg, uint8_t *__restrict__ seedcls)
+{
+    for (long long i = gtid(); i < ns; i += gstride()) {
+        const int v = S[i];
+        const int lb = label[v];
+        int c = bg;
+        if (lb >= 0 && lb <= maxl && l2c[lb] >= 0) c = l2c[lb];
+        seedcls[v] = (uint8_t)c;
+    }
+}
+
+__global__ void k_labels_from_init(long long n, const int32_t *__restrict__ label, const int16_t *__restrict__ l2c,
+                                   int maxl, int bg, const int32_t *__restrict__ cls_label_dev, int32_t *__restrict__ out)
+{
+    for (long long p = gtid(); p < n; p += gstride()) {
+        const int lb = label[p];
+        int c = bg;
+        if (lb >= 0 && lb <= maxl && l2c[lb] >= 0) c = l2c[lb];
+        out[p] = cls_label_dev[c];
+    }
+}
+
+__global__ void k_memb_out2(long long n, const float *__restrict__ f, int bg, float *__restrict__ out)
+{
+    for (long long p = gtid(); p < n; p += gstride()) {
+        const float v = f[p];
+        out[bg * n + p] = v;
+        out[(1 - bg) * n + p] = 1.0f - v;
+    }
+}
+
+template <int NADJ, typename IT>
+void launch_iter(roift_ctx &c, const void *img, const RelaxArgs &a, const float *fin, float *fout, const uint8_t *seedcls,
+                 int32_t *label_out)
+{
+    const unsigned g = grid_for(a.d.n, 256, 1 << 16);
+    if (a.nc == 2)
+        hipLaunchKernelGGL((k_relax2<NADJ, IT>), dim3(g), dim3(256), 0, c.stream, (const IT *)img, a, fin, fout, seedcls,
+                           label_out);
+    else
+        hipLaunchKernelGGL((k_relaxn<NADJ, IT>), dim3(g), dim3(256), 0, c.stream, (const IT *)img, a, fin, fout, seedcls,
+                           label_out);
+    ROIFT_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, const RelaxSpec &rs,
+              const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter, float *d_memb, int32_t *nclasses_out,
+              int32_t maxI)
+{
+    ROIFT_REQUIRE(nadj == 7 || nadj == 27, ROIFT_EUNSUPPORTED, "adjacency must be Spheric(1.0) or Spheric(1.75)");
+    ROIFT_REQUIRE(n_seeds > 0, ROIFT_EINVAL, "relaxation needs seeds (the S==NULL form is ORelax_1)");
+    ROIFT_REQUIRE(niter >= 0, ROIFT_EINVAL, "negative iteration count");
+    const long long n = d.n;
+    hipStream_t st = c.stream;
+    // --- classes: seed labels present, plus 0 (gft_ift.cpp:8684-8720)
+    int32_t *sl = buf<int32_t>(c, "relax.seedlab", n_seeds);
+    int *info = buf<int>(c, "relax.info", 2);
+    const int info0[2] = {0, 0};
+    ROIFT_HIP_CHECK(hipMemcpyAsync(info, info0, sizeof(info0), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_seed_labels, dim3(grid_for(n_seeds)), dim3(256), 0, st, d_seeds, n_seeds, n, d_label, sl, info);
+    ROIFT_HIP_CHECK(hipGetLastError());
+    int hinfo[2];
+    ROIFT_HIP_CHECK(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
+    ROIFT_HIP_CHECK(hipStreamSynchronize(st));
+    ROIFT_REQUIRE(hinfo[1] == 0, ROIFT_EINVAL, "seed index out of range");
+    const int maxl = std::max(0, hinfo[0]);
+    ROIFT_REQUIRE(maxl < (1 << 20), ROIFT_EUNSUPPORTED, "seed labels above 2^20");
+    uint8_t *present = buf<uint8_t>(c, "relax.present", (size_t)maxl + 1);
+    fill<uint8_t>(c, present, maxl + 1, 0);
+    hipLaunchKernelGGL(k_mark_present, dim3(grid_for(n_seeds)), dim3(256), 0, st, sl, n_seeds, maxl, present);
+    std::vector<uint8_t> hp((size_t)maxl + 1);
+    ROIFT_HIP_CHECK(hipMemcpyAsync(hp.data(), present, hp.size(), hipMemcpyDeviceToHost, st));
+    ROIFT_HIP_CHECK(hipStreamSynchronize(st));
+    hp[0] = 1;
+    std::vector<int> cls;
+    std::vector<int16_t> l2c((size_t)maxl + 1, -1);
+    for (int l = 0; l <= maxl; l++)
+        if (hp[l]) {
+            l2c[l] = (int16_t)cls.size();
+            cls.push_back(l);
+        }
+    const int nc = (int)cls.size();
+    ROIFT_REQUIRE(nc <= NC_MAX, ROIFT_EUNSUPPORTED, "more than 8 relaxation classes");
+    if (nclasses_out) *nclasses_out = nc;
+    const int bg = l2c[0];  // label 0 is always class 0
+
+    RelaxArgs a{};
+    a.d = d;
+    a.per = rs.per;
+    a.use_pc = rs.use_per_class;
+    a.nc = nc;
+    a.bg = bg;
+    float max_abs = std::fabs(rs.per);
+    if (rs.use_per_class) {
+        max_abs = 0.f;
+        for (int k = 0; k < nc; k++) {
+            const int L = cls[k];
+            a.pcm[k] = (L <= rs.max_label) ? rs.per_class[L] : 0.0f;
+            max_abs = std::max(max_abs, std::fabs(a.pcm[k]));
+        }
+    }
+    // float Wmax = (float)Imax; Wmax *= (1.0 + |per|/100.0)   (gft_ift.cpp:8733-8734)
+    a.Wmax = (float)((double)(float)maxI * (1.0 + (double)max_abs / 100.0));
+    for (int k = 0; k < 27; k++) a.Dpq[k] = 1.0f;
+    if (nadj == 27)
+        for (int k = 1; k < 27; k++) {
+            const int ax = Adjacency<27>::dx(k), ay = Adjacency<27>::dy(k), az = Adjacency<27>::dz(k);
+            a.Dpq[k] = sqrtf((float)(ax * ax) * 1.0f * 1.0f + (float)(ay * ay) * 1.0f * 1.0f +
+                             (float)(az * az) * 1.0f * 1.0f) / 1.0f;
+        }
+    for (int k = 0; k < nc; k++) a.cls_label[k] = cls[k];
+
+    int16_t *dl2c = buf<int16_t>(c, "relax.l2c", (size_t)maxl + 1);
+    ROIFT_HIP_CHECK(hipMemcpyAsync(dl2c, l2c.data(), l2c.size() * sizeof(int16_t), hipMemcpyHostToDevice, st));
+    const long long chans = (nc == 2) ? 1 : nc;
+    float *f0 = buf<float>(c, "relax.f0", (size_t)(chans * n));
+    float *f1 = buf<float>(c, "relax.f1", (size_t)(chans * n));
+    uint8_t *seedcls = buf<uint8_t>(c, "relax.seedcls", n);
+    const unsigned G = grid_for(n, 256, 8192);
+    hipLaunchKernelGGL(k_relax_init, dim3(G), dim3(256), 0, st, n, d_label, dl2c, maxl, nc, bg, f0, seedcls);
+    hipLaunchKernelGGL(k_seed_classes, dim3(grid_for(n_seeds)), dim3(256), 0, st, d_seeds, n_seeds, d_label, dl2c, maxl,
+                       bg, seedcls);
+    ROIFT_HIP_CHECK(hipGetLastError());
+    if (niter == 0) {
+        int32_t *cl = buf<int32_t>(c, "relax.clslab", nc);
+        ROIFT_HIP_CHECK(hipMemcpyAsync(cl, cls.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_labels_from_init, dim3(G), dim3(256), 0, st, n, d_label, dl2c, maxl, bg, cl, d_label);
+    }
+    float *fin = f0, *fout = f1;
+    for (int it = 0; it < niter; it++) {
+        int32_t *lo = (it == niter - 1) ? d_label : nullptr;
+        const bool i16 = image_dtype == ROIFT_I16;
+        if (nadj == 7) {
+            if (i16) launch_iter<7, int16_t>(c, d_image, a, fin, fout, seedcls, lo);
+            else launch_iter<7, int32_t>(c, d_image, a, fin, fout, seedcls, lo);
+        } else {
+            if (i16) launch_iter<27, int16_t>(c, d_image, a, fin, fout, seedcls, lo);
+            else launch_iter<27, int32_t>(c, d_image, a, fin, fout, seedcls, lo);
+        }
+        std::swap(fin, fout);
+    }
+    if (d_memb) {
+        if (nc == 2)
+            hipLaunchKernelGGL(k_memb_out2, dim3(G), dim3(256), 0, st, n, fin, bg, d_memb);
+        else
+            ROIFT_HIP_CHECK(hipMemcpyAsync(d_memb, fin, sizeof(float) * nc * n, hipMemcpyDeviceToDevice, st));
+        ROIFT_HIP_CHECK(hipGetLastError());
+    }
+    ROIFT_HIP_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+}  // namespace roift

@@ -1,0 +1,154 @@
+// weights.hip -- fused oriented arc-weight builder for gfx950.
+//
+// Replaces the weight that OIFT_Multi recomputes inside its pop loop
+// (gft_ift.cpp:965-973): w = |I_p - I_q| scaled by (1 +- per/100) in double,
+// truncated toward zero.  One pass reads the image once (int16 or int32) and
+// writes, for every forward neighbour slot, the +per ("object") weight of both
+// orientations of the edge.  The -per ("background") weight of an arc is the
+// object weight of the reverse arc, so two values per undirected edge carry
+// both polarities.  The same pass reduces the image extrema.
+//
+// Bound: HBM.  Algorithmic bytes per voxel = sizeof(image) + 2*E*sizeof(weight)
+// (E = 3 forward edges for 6-adjacency, 13 for 26).
+#include "ctx.hpp"
+
+namespace roift {
+
+template <int NADJ, typename IT, typename WT>
+__global__ __launch_bounds__(256) void k_weights(const IT *__restrict__ img, Dims d, double fp, double fm,
+                                                 WT *__restrict__ wa, WT *__restrict__ wb, int *__restrict__ ext,
+                                                 uint32_t *__restrict__ overflow)
+{
+    using A = Adjacency<NADJ>;
+    constexpr uint32_t WMAXV = sizeof(WT) == 2 ? 0xFFFFu : 0xFFFFFFFFu;
+    const long long xy = (long long)d.xs * d.ys;
+    int lmax = INT32_MIN, lmin = INT32_MAX;
+    uint32_t ovf = 0;
+    for (long long p = gtid(); p < d.n; p += gstride()) {
+        const int x = (int)(p % d.xs), y = (int)((p / d.xs) % d.ys), z = (int)(p / xy);
+        const int Ip = (int)img[p];
+        lmax = max(lmax, Ip);
+        lmin = min(lmin, Ip);
+#pragma unroll
+        for (int s = 0; s < A::half; s++) {
+            const int i = A::half + 1 + s;
+            const int qx = x + A::dx(i), qy = y + A::dy(i), qz = z + A::dz(i);
+            uint32_t a = WMAXV, b = WMAXV;
+            if (in_volume(d, qx, qy, qz)) {
+                const long long q = qx + (long long)qy * d.xs + (long long)qz * xy;
+                const int Iq = (int)img[q];
+                const int dd = abs(Ip - Iq);
+                // w_obj(p->q): per_pq=+per; w_obj(q->p): roles of p and q swapped
+                const int up = (int)((double)dd * fp), dn = (int)((double)dd * fm);
+                a = (uint32_t)(Ip > Iq ? up : (Ip < Iq ? dn : dd));
+                b = (uint32_t)(Iq > Ip ? up : (Iq < Ip ? dn : dd));
+                if (a >= WMAXV || b >= WMAXV) ovf = 1;
+            }
+            wa[s * d.n + p] = (WT)a;
+            wb[s * d.n + p] = (WT)b;
+        }
+    }
+    // block reduction of the extrema, then one atomic per block
+    __shared__ int smax[8], smin[8];
+    for (int o = 32; o > 0; o >>= 1) {
+        lmax = max(lmax, __shfl_xor(lmax, o));
+        lmin = min(lmin, __shfl_xor(lmin, o));
+    }
+    const int wid = threadIdx.x / 64;
+    if (__lane_id() == 0) { smax[wid] = lmax; smin[wid] = lmin; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x / 64); w++) { lmax = max(lmax, smax[w]); lmin = min(lmin, smin[w]); }
+        atomicMax(&ext[0], lmax);
+        atomicMin(&ext[1], lmin);
+    }
+    if (ovf) atomicOr(overflow, 1u);
+}
+
+template <typename IT>
+__global__ __launch_bounds__(256) void k_extrema(const IT *__restrict__ img, long long n, int *__restrict__ ext)
+{
+    int lmax = INT32_MIN, lmin = INT32_MAX;
+    for (long long p = gtid(); p < n; p += gstride()) {
+        const int v = (int)img[p];
+        lmax = max(lmax, v);
+        lmin = min(lmin, v);
+    }
+    __shared__ int smax[8], smin[8];
+    for (int o = 32; o > 0; o >>= 1) {
+        lmax = max(lmax, __shfl_xor(lmax, o));
+        lmin = min(lmin, __shfl_xor(lmin, o));
+    }
+    const int wid = threadIdx.x / 64;
+    if (__lane_id() == 0) { smax[wid] = lmax; smin[wid] = lmin; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x / 64); w++) { lmax = max(lmax, smax[w]); lmin = min(lmin, smin[w]); }
+        atomicMax(&ext[0], lmax);
+        atomicMin(&ext[1], lmin);
+    }
+}
+
+namespace {
+
+template <int NADJ, typename IT, typename WT>
+void launch_weights(roift_ctx &c, const Dims &d, const void *img, double fp, double fm, void *wa, void *wb, int *ext,
+                    uint32_t *ovf)
+{
+    hipLaunchKernelGGL((k_weights<NADJ, IT, WT>), dim3(grid_for(d.n, 256, 1 << 16)), dim3(256), 0, c.stream,
+                       (const IT *)img, d, fp, fm, (WT *)wa, (WT *)wb, ext, ovf);
+    ROIFT_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,
+                         void *d_wa, void *d_wb, int weight_dtype)
+{
+    ROIFT_REQUIRE(nadj == 7 || nadj == 27, ROIFT_EUNSUPPORTED, "adjacency must be Spheric(1.0) or Spheric(1.75)");
+    ROIFT_REQUIRE(image_dtype == ROIFT_I16 || image_dtype == ROIFT_I32, ROIFT_EINVAL, "image dtype must be I16 or I32");
+    ROIFT_REQUIRE(weight_dtype == ROIFT_U16 || weight_dtype == ROIFT_U32, ROIFT_EINVAL, "weight dtype must be U16 or U32");
+    int *ext = buf<int>(c, "w.ext", 4);
+    uint32_t *ovf = reinterpret_cast<uint32_t *>(ext + 2);
+    const int init[4] = {INT32_MIN, INT32_MAX, 0, 0};
+    ROIFT_HIP_CHECK(hipMemcpyAsync(ext, init, sizeof(init), hipMemcpyHostToDevice, c.stream));
+    // per_pq / 100.0 in double, as the reference's "w *= (1.0 +- per_pq/100.0)"
+    const double fp = 1.0 + (double)per / 100.0, fm = 1.0 - (double)per / 100.0;
+    const bool i16 = image_dtype == ROIFT_I16, w16 = weight_dtype == ROIFT_U16;
+    if (nadj == 7) {
+        if (i16 && w16) launch_weights<7, int16_t, uint16_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
+        else if (i16) launch_weights<7, int16_t, uint32_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
+        else if (w16) launch_weights<7, int32_t, uint16_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
+        else launch_weights<7, int32_t, uint32_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
+    } else {
+        if (i16 && w16) launch_weights<27, int16_t, uint16_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
+        else if (i16) launch_weights<27, int16_t, uint32_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
+        else if (w16) launch_weights<27, int32_t, uint16_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
+        else launch_weights<27, int32_t, uint32_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
+    }
+    int host[4];
+    ROIFT_HIP_CHECK(hipMemcpyAsync(host, ext, sizeof(host), hipMemcpyDeviceToHost, c.stream));
+    ROIFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+    ROIFT_REQUIRE(host[2] == 0, ROIFT_EUNSUPPORTED, "arc weight exceeds the weight dtype range (use U32 weights)");
+    return WeightInfo{host[0], host[1]};
+}
+
+WeightInfo image_extrema(roift_ctx &c, const Dims &d, const void *d_image, int image_dtype)
+{
+    int *ext = buf<int>(c, "w.ext", 4);
+    const int init[4] = {INT32_MIN, INT32_MAX, 0, 0};
+    ROIFT_HIP_CHECK(hipMemcpyAsync(ext, init, sizeof(init), hipMemcpyHostToDevice, c.stream));
+    if (image_dtype == ROIFT_I16)
+        hipLaunchKernelGGL(k_extrema<int16_t>, dim3(grid_for(d.n, 256, 4096)), dim3(256), 0, c.stream,
+                           (const int16_t *)d_image, d.n, ext);
+    else
+        hipLaunchKernelGGL(k_extrema<int32_t>, dim3(grid_for(d.n, 256, 4096)), dim3(256), 0, c.stream,
+                           (const int32_t *)d_image, d.n, ext);
+    ROIFT_HIP_CHECK(hipGetLastError());
+    int host[4];
+    ROIFT_HIP_CHECK(hipMemcpyAsync(host, ext, sizeof(host), hipMemcpyDeviceToHost, c.stream));
+    ROIFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+    return WeightInfo{host[0], host[1]};
+}
+
+}  // namespace roift

@@ -105,7 +105,8 @@ def plumbing_case_2d(seed: int = 1) -> Case:
     img = np.clip(np.rint(img), 0, 255)
     img = np.rint(img * 1000.0 / 255.0).astype(np.int32)[None]
     internal = np.array([x_ + y_ * n for y_ in range(253, 260) for x_ in range(253, 260)], np.int64)
-    border = _face_seeds((1, n, n))
+    yy, xx = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    border = np.flatnonzero(((xx == 0) | (xx == n - 1) | (yy == 0) | (yy == n - 1)).ravel())
     border = border[~np.isin(border, internal)]
     seeds = np.concatenate([internal, border])
     labels = np.concatenate([np.ones(internal.size, np.int32), np.zeros(border.size, np.int32)])

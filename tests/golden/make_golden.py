@@ -1,0 +1,81 @@
+"""Writes the golden fixtures in this directory from the compiled reference.
+
+Runs gft::ift::OIFT_Multi[_PerClass] and ORelax_1_Multi[_PerClass] of the
+reference (oracle/_ref/libroift_ref.so, built by `make -C oracle ref` from the
+reference sources) on small deterministic inputs and stores inputs and
+outputs as .npz.  The reference holds no fixtures of its own, so these are
+the reference-produced known answers the oracle and the GPU path are pinned
+to.  Re-run only if the inputs below change:
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle as O  # noqa: E402
+from roift_amd import phantom as P  # noqa: E402
+
+
+def border_seeds_2d(n):
+    idx = [x + y * n for y in range(n) for x in range(n) if x in (0, n - 1) or y in (0, n - 1)]
+    return np.array(idx, np.int64)
+
+
+def cases():
+    # 1. the reference's CPU-runnable 2-D case: 512^2, 4-adjacency (zsize=1), per=+50, 10 iterations
+    c = P.plumbing_case_2d()
+    yield "plumbing2d_512", dict(image=c.image, seeds=c.seeds, seed_labels=c.seed_labels, adj_r=1.0, per=50.0,
+                                 per_class=None, niter=10)
+    # 2. small 3-D ellipsoid phantoms, 6-adjacency, natural ties
+    for i, (shape, seed, per) in enumerate([((12, 14, 16), 11, 50.0), ((16, 18, 20), 12, 50.0),
+                                            ((10, 20, 24), 13, -30.0), ((14, 14, 14), 14, 0.0)]):
+        c = P.small_case(shape, seed, per=per, top=255)
+        yield f"ell6_{i}", dict(image=c.image, seeds=c.seeds, seed_labels=c.seed_labels, adj_r=1.0, per=per,
+                                per_class=None, niter=8)
+    # 3. 26-adjacency
+    c = P.small_case((12, 12, 14), 21, per=50.0, top=255)
+    yield "ell26_0", dict(image=c.image, seeds=c.seeds, seed_labels=c.seed_labels, adj_r=1.75, per=50.0,
+                          per_class=None, niter=6)
+    # 4. three labels, per-class polarity {-50, +50, +50}; the third label seeds a corner blob
+    c = P.small_case((14, 16, 18), 31, per=50.0, top=255)
+    zs, ys, xs = c.image.shape
+    extra = np.array([3 + 3 * xs + 3 * xs * ys, 4 + 3 * xs + 3 * xs * ys], np.int64)
+    keep = ~np.isin(c.seeds, extra)
+    seeds = np.concatenate([c.seeds[keep][:27], extra, c.seeds[keep][27:]])
+    labels = np.concatenate([c.seed_labels[keep][:27], [2, 2], c.seed_labels[keep][27:]]).astype(np.int32)
+    yield "perclass3", dict(image=c.image, seeds=seeds, seed_labels=labels, adj_r=1.0, per=0.0,
+                            per_class=np.array([-50.0, 50.0, 50.0], np.float32), niter=5)
+    # 5. constant image (all weights zero): labels follow the FIFO order of S
+    img = np.full((6, 7, 8), 100, np.int16)
+    s = np.array([0, 8 * 7 * 6 - 1, 3 + 3 * 8 + 3 * 56], np.int64)
+    yield "constant", dict(image=img, seeds=s, seed_labels=np.array([0, 0, 1], np.int32), adj_r=1.0, per=50.0,
+                           per_class=None, niter=3)
+
+
+def main():
+    if O.ref_lib() is None:
+        raise SystemExit("oracle/_ref/libroift_ref.so is not built (make -C oracle ref)")
+    for name, c in cases():
+        img = np.ascontiguousarray(c["image"], np.int32)
+        lab0 = np.full(img.size, -1, np.int32)
+        lab0[c["seeds"]] = c["seed_labels"]
+        lab0 = lab0.reshape(img.shape)
+        oift = O.ref_oift(img, c["seeds"], lab0, c["adj_r"], c["per"], c["per_class"])
+        relax = O.ref_relax(img, c["seeds"], oift, c["niter"], c["adj_r"], c["per"], c["per_class"])
+        np.savez_compressed(
+            os.path.join(HERE, f"{name}.npz"), image=img.astype(np.int16), seeds=c["seeds"].astype(np.int32),
+            seed_labels=c["seed_labels"].astype(np.int32), adj_r=np.float32(c["adj_r"]), per=np.float32(c["per"]),
+            per_class=(np.zeros(0, np.float32) if c["per_class"] is None else c["per_class"]),
+            niter=np.int32(c["niter"]), oift_label=oift.astype(np.int8 if oift.max() < 127 else np.int32),
+            relax_label=relax.astype(np.int8))
+        print(name, img.shape, "obj voxels", int((oift > 0).sum()), int((relax > 0).sum()))
+
+
+if __name__ == "__main__":
+    main()

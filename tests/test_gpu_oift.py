@@ -1,0 +1,225 @@
+"""GPU parity of the oriented weight builder and the OIFT (f_max) against the oracle.
+
+Bar: bit-exact labels and cost maps (integer work), on tied and tie-free maps.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import roift_amd as R
+from roift_amd import gft, phantom as P
+from conftest import golden_cases
+
+pytestmark = pytest.mark.gpu
+
+INT32_MAX = 2**31 - 1
+
+
+def dev(a, dtype=None):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to("cuda:0")
+
+
+def run_gpu(ctx, image, seeds, label0, adj_r, per, per_class=None, idtype="i16"):
+    import torch
+    img = dev(image.astype(np.int16 if idtype == "i16" else np.int32))
+    lab = dev(label0.astype(np.int32))
+    cost = ctx.oift_fmax(img, dev(seeds.astype(np.int32)), lab, adj_r, per, per_class)
+    torch.cuda.synchronize()
+    return lab.cpu().numpy(), cost.cpu().numpy()
+
+
+@pytest.mark.parametrize("adj,per,idt,wdt", [(1.0, 50.0, "i16", "u16"), (1.0, -30.0, "i32", "u16"),
+                                             (1.75, 50.0, "i16", "u32"), (1.75, 12.5, "i32", "u32")])
+def test_weight_builder_matches_oracle(ctx, adj, per, idt, wdt):
+    c = P.small_case((9, 10, 11), 3, per=per, adj_r=adj, top=4095)
+    img = dev(c.image.astype(np.int16 if idt == "i16" else np.int32))
+    wa, wb, mx, mn = ctx.weights(img, adj, per, wdt)
+    oa, ob = O.edge_weights(c.image, adj, per)
+    ga = wa.cpu().numpy().view(np.uint16 if wdt == "u16" else np.uint32).astype(np.uint32)
+    gb = wb.cpu().numpy().view(np.uint16 if wdt == "u16" else np.uint32).astype(np.uint32)
+    inside = oa != 0xFFFFFFFF
+    assert np.array_equal(ga[inside], oa[inside]) and np.array_equal(gb[inside], ob[inside])
+    top = 0xFFFF if wdt == "u16" else 0xFFFFFFFF
+    assert (ga[~inside] == top).all() and (gb[~inside] == top).all()
+    assert mx == int(c.image.max()) and mn == int(c.image.min())
+
+
+@pytest.mark.parametrize("case", golden_cases(), ids=lambda c: c["name"])
+def test_oift_matches_reference_golden(ctx, case):
+    lab, cost = run_gpu(ctx, case["image"], case["seeds"], case["label0"], case["adj_r"], case["per"],
+                        case["per_class"])
+    assert np.array_equal(lab, case["oift_label"].astype(np.int32))
+    o = O.oift_image(case["image"], case["seeds"], case["label0"], case["adj_r"], case["per"], case["per_class"])
+    assert np.array_equal(cost, o["cost"])
+
+
+CASES = [((12, 13, 14), 0, 1.0, 50.0, 4095, "i16"), ((7, 31, 9), 1, 1.0, -50.0, 255, "i32"),
+         ((16, 16, 16), 2, 1.0, 0.0, 255, "i16"), ((5, 40, 33), 3, 1.0, 100.0, 4095, "i16"),
+         ((13, 11, 17), 4, 1.75, 50.0, 4095, "i16"), ((9, 9, 9), 5, 1.75, -20.0, 255, "i32"),
+         ((1, 48, 64), 6, 1.0, 50.0, 1000, "i16"), ((24, 20, 30), 7, 1.0, 37.5, 4095, "i16"),
+         ((2, 3, 70), 8, 1.0, 50.0, 255, "i16"), ((30, 28, 26), 9, 1.0, 50.0, 65535 // 2, "i32")]
+
+
+@pytest.mark.parametrize("shape,seed,adj,per,top,idt", CASES)
+def test_oift_matches_oracle(ctx, shape, seed, adj, per, top, idt):
+    c = P.small_case(shape, seed, per=per, adj_r=adj, top=top)
+    lab0 = c.label_init()
+    lab, cost = run_gpu(ctx, c.image, c.seeds, lab0, adj, per, idtype=idt)
+    o = O.oift_image(c.image, c.seeds, lab0, adj, per)
+    assert np.array_equal(lab, o["label"])
+    assert np.array_equal(cost, o["cost"])
+
+
+def test_oift_per_class_three_labels(ctx):
+    c = P.small_case((18, 20, 22), 11, per=50.0, top=4095)
+    zs, ys, xs = c.image.shape
+    extra = np.array([3 + 4 * xs + 5 * xs * ys, 4 + 4 * xs + 5 * xs * ys, 15 + 14 * xs + 12 * xs * ys], np.int64)
+    keep = ~np.isin(c.seeds, extra)
+    seeds = np.concatenate([extra[:2], c.seeds[keep], extra[2:]])
+    labels = np.concatenate([[2, 2], c.seed_labels[keep], [3]]).astype(np.int32)
+    lab0 = np.full(c.image.size, -1, np.int32)
+    lab0[seeds] = labels
+    lab0 = lab0.reshape(c.image.shape)
+    for pc in (np.array([-50, 50, 50, 50], np.float32), np.array([30, -30, -30, 30], np.float32)):
+        lab, cost = run_gpu(ctx, c.image, seeds, lab0, 1.0, 0.0, pc)
+        o = O.oift_image(c.image, seeds, lab0, 1.0, per_class=pc)
+        assert np.array_equal(lab, o["label"]) and np.array_equal(cost, o["cost"])
+    # OIFT_Multi form with four labels: every label != 0 uses +per
+    lab, cost = run_gpu(ctx, c.image, seeds, lab0, 1.0, 50.0)
+    o = O.oift_image(c.image, seeds, lab0, 1.0, 50.0)
+    assert np.array_equal(lab, o["label"]) and np.array_equal(cost, o["cost"])
+
+
+def test_oift_frozen_labels_and_unreached_voxels(ctx):
+    """Preset non-seed labels are never popped and block propagation; voxels walled
+    off by them keep NIL and cost INT32_MAX, as in the reference (gft_ift.cpp:930-943)."""
+    c = P.small_case((10, 12, 14), 12, per=50.0, top=255)
+    lab0 = c.label_init().copy()
+    lab0[:, :, 3] = 2            # a frozen wall of label 2 (not in S) at x = 3
+    keep = (c.seeds % 14) > 3    # no seeds left of the wall: x < 3 stays unreached
+    seeds, labels = c.seeds[keep], c.seed_labels[keep]
+    lab0.ravel()[seeds] = labels
+    lab, cost = run_gpu(ctx, c.image, seeds, lab0, 1.0, 50.0)
+    o = O.oift_image(c.image, seeds, lab0, 1.0, 50.0)
+    assert np.array_equal(lab, o["label"]) and np.array_equal(cost, o["cost"])
+    assert (lab[:, :, 3] == 2).all() and (cost[:, :, 3] == INT32_MAX).all()
+    assert (lab[:, :, :3] == -1).all() and (cost[:, :, :3] == INT32_MAX).all()
+
+
+def test_oift_wide_weights_use_u32(ctx):
+    """Wmax above 65534 switches the weight builder to 32-bit weights."""
+    c = P.small_case((10, 11, 12), 14, per=50.0, top=4095)
+    img = c.image.astype(np.int32) * 16        # max 65520 -> Wmax = 98280
+    lab, cost = run_gpu(ctx, img, c.seeds, c.label_init(), 1.0, 50.0, idtype="i32")
+    o = O.oift_image(img, c.seeds, c.label_init(), 1.0, 50.0)
+    assert np.array_equal(lab, o["label"]) and np.array_equal(cost, o["cost"])
+
+
+def tie_free_weights(shape, adj, seed):
+    """Tie-free oriented weights: the dense rank of (|dI| scaled by 1 +- 0.5, hash) over
+    every (edge, orientation), so no two arcs share a value."""
+    rng = np.random.default_rng(seed)
+    c = P.small_case(shape, seed, per=50.0, adj_r=adj, top=4095)
+    wa, wb = O.edge_weights(c.image, adj, 50.0)
+    inside = wa != 0xFFFFFFFF
+    base = np.concatenate([wa[inside], wb[inside]]).astype(np.int64)
+    perm = rng.permutation(base.size)
+    order = np.lexsort((perm, base))
+    rank = np.empty(base.size, np.int64)
+    rank[order] = np.arange(base.size)
+    na = wa.copy()
+    nb = wb.copy()
+    m = int(inside.sum())
+    na[inside] = rank[:m]
+    nb[inside] = rank[m:]
+    return c, na, nb
+
+
+@pytest.mark.parametrize("shape,adj,seed", [((10, 11, 12), 1.0, 0), ((8, 9, 10), 1.75, 1), ((20, 22, 24), 1.0, 2)])
+def test_oift_tie_free_edges_bit_exact(ctx, shape, adj, seed):
+    import torch
+    c, wa, wb = tie_free_weights(shape, adj, seed)
+    lab0 = c.label_init()
+    lab = dev(lab0)
+    cost = ctx.oift_edges(dev(wa.view(np.int32)), dev(wb.view(np.int32)), dev(c.seeds.astype(np.int32)), lab, adj)
+    torch.cuda.synchronize()
+    o = O.oift_edges(wa, wb, c.seeds, lab0, adj)
+    assert np.array_equal(lab.cpu().numpy(), o["label"])
+    assert np.array_equal(cost.cpu().numpy(), o["cost"])
+
+
+def test_oift_input_errors(ctx):
+    c = P.small_case((6, 7, 8), 0, top=255)
+    lab0 = c.label_init()
+    bad = [np.concatenate([c.seeds, [c.image.size]]),           # out of range
+           np.concatenate([c.seeds, c.seeds[:1]])]             # duplicate
+    for s in bad:
+        with pytest.raises(R.RoiftError) as e:
+            run_gpu(ctx, c.image, s, lab0, 1.0, 50.0)
+        assert e.value.code == R.EINVAL
+    nil_seed = lab0.copy()
+    nil_seed.ravel()[c.seeds[0]] = -1
+    with pytest.raises(R.RoiftError) as e:
+        run_gpu(ctx, c.image, c.seeds, nil_seed, 1.0, 50.0)
+    assert e.value.code == R.EINVAL
+    neg = c.image.astype(np.int32) - 10
+    with pytest.raises(R.RoiftError):
+        run_gpu(ctx, neg, c.seeds, lab0, 1.0, 50.0, idtype="i32")
+    # no seeds: nothing is popped
+    lab, cost = run_gpu(ctx, c.image, np.zeros(0, np.int64), lab0, 1.0, 50.0)
+    assert np.array_equal(lab, lab0) and (cost == INT32_MAX).all()
+
+
+def test_oift_repeatable(ctx):
+    c = P.small_case((20, 21, 22), 13, per=50.0)
+    a = run_gpu(ctx, c.image, c.seeds, c.label_init(), 1.0, 50.0)
+    b = run_gpu(ctx, c.image, c.seeds, c.label_init(), 1.0, 50.0)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("n", [64, 128])
+def test_oift_headline_phantom_scaled(ctx, n):
+    c = P.headline_case(n)
+    lab, cost = run_gpu(ctx, c.image, c.seeds, c.label_init(), 1.0, 50.0)
+    o = O.oift_image(c.image, c.seeds, c.label_init(), 1.0, 50.0)
+    assert np.array_equal(lab, o["label"]) and np.array_equal(cost, o["cost"])
+
+
+def test_oift_headline_full_size(ctx):
+    """256^3 headline workload: cost map and labels bit-exact against the oracle."""
+    c = P.headline_case(256)
+    lab, cost = run_gpu(ctx, c.image, c.seeds, c.label_init(), 1.0, 50.0)
+    o = O.oift_image(c.image, c.seeds, c.label_init(), 1.0, 50.0)
+    assert np.array_equal(cost, o["cost"])
+    assert np.array_equal(lab, o["label"])
+
+
+def test_legacy_c_entry_point(ctx):
+    c = P.small_case((14, 15, 16), 21, per=50.0, top=4095)
+    zs, ys, xs = c.image.shape
+    lab = c.label_init().copy()
+    pc = np.array([-50.0, 50.0], np.float32)
+    steps = R.oift_fmax_gpu_run(c.image, lab, xs, ys, zs, pc, c.seeds)
+    assert steps > 0
+    o = O.oift_image(c.image, c.seeds, c.label_init(), 1.0, per_class=pc)
+    assert np.array_equal(lab, o["label"])
+
+
+def test_gft_mirror_api(ctx):
+    case = [g for g in golden_cases() if g["name"] == "ell6_1"][0]
+    A = gft.AdjRel3.Spheric(1.0)
+    scn = gft.Scene32.FromArray(case["image"].astype(np.int32))
+    label = gft.Scene32.Create(scn)
+    gft.Scene32.Fill(label, -1)
+    S = np.concatenate([[len(case["seeds"])], case["seeds"]]).astype(np.int32)
+    label.data[case["seeds"]] = case["seed_labels"]
+    gft.ift.OIFT_Multi(A, scn, case["per"], S, label)
+    assert np.array_equal(label.volume(), case["oift_label"].astype(np.int32))
+    gft.ift.ORelax_1_Multi(A, scn, case["per"], S, label, case["niter"])
+    r64 = O.relax(case["image"], case["seeds"], case["oift_label"], case["niter"], 1.0, case["per"], fp64=True)
+    near = np.abs(r64["memb"][0] - 0.5) < 1e-5
+    assert np.array_equal(label.volume()[~near], case["relax_label"].astype(np.int32)[~near])

@@ -1,0 +1,249 @@
+"""Pins for the CPU oracle of OIFT_Multi (oracle/oracle.c) -- CPU only.
+
+Each check is chosen to fail on a plausible mistake in the restatement:
+  golden / _ref      -- any deviation from the reference (tie order, weight rounding, polarity sign)
+  Kruskal minimax    -- wrong cost formula, wrong neighbour offsets (per = 0 closed form)
+  Kruskal labels     -- wrong label propagation on distinct weights (per = 0)
+  FIFO BFS           -- wrong tie order (constant image reduces to BFS in S order)
+  invariants         -- pred/cost inconsistency, cycles, Bellman violations
+  two queues         -- Dial bucket queue vs (value, seq) heap on the same weights
+"""
+from collections import deque
+
+import numpy as np
+import pytest
+
+import oracle as O
+from roift_amd import phantom as P
+from conftest import golden_cases
+
+ADJ6 = [(0, 0, 0), (0, 0, -1), (0, -1, 0), (-1, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)]
+
+
+def test_spheric_order_matches_gft():
+    assert [tuple(r) for r in O.spheric(1.0)] == ADJ6
+    a26 = [tuple(r) for r in O.spheric(1.75)]
+    raster = [(x, y, z) for z in (-1, 0, 1) for y in (-1, 0, 1) for x in (-1, 0, 1)]
+    raster.remove((0, 0, 0))
+    assert a26 == [(0, 0, 0)] + raster
+    # i and n-i are opposite displacements
+    for A in (O.spheric(1.0), O.spheric(1.75)):
+        n = len(A)
+        for i in range(1, n):
+            assert tuple(-A[i]) == tuple(A[n - i])
+
+
+@pytest.mark.parametrize("case", golden_cases(), ids=lambda c: c["name"])
+def test_oracle_matches_reference_golden(case):
+    o = O.oift_image(case["image"], case["seeds"], case["label0"], case["adj_r"], case["per"], case["per_class"])
+    assert np.array_equal(o["label"], case["oift_label"].astype(np.int32))
+    r = O.relax(case["image"], case["seeds"], o["label"], case["niter"], case["adj_r"], case["per"],
+                case["per_class"])
+    assert np.array_equal(r["label"], case["relax_label"].astype(np.int32))
+
+
+@pytest.mark.skipif(O.ref_lib() is None, reason="oracle/_ref not built (no reference checkout)")
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_matches_compiled_reference(seed):
+    rng = np.random.default_rng(100 + seed)
+    shape = tuple(int(v) for v in rng.integers(5, 22, 3))
+    adj_r = [1.0, 1.75][seed % 2]
+    per = float(rng.choice([50.0, -50.0, 30.0, 0.0, 100.0, 12.5]))
+    c = P.small_case(shape, seed, per=per, adj_r=adj_r, top=int(rng.choice([255, 4095])))
+    lab0 = c.label_init()
+    o = O.oift_image(c.image, c.seeds, lab0, adj_r, per)
+    assert np.array_equal(o["label"], O.ref_oift(c.image, c.seeds, lab0, adj_r, per))
+    r = O.relax(c.image, c.seeds, o["label"], 4, adj_r, per)
+    assert np.array_equal(r["label"], O.ref_relax(c.image, c.seeds, o["label"], 4, adj_r, per))
+    pc = np.array([-per, per, per], np.float32)
+    o2 = O.oift_image(c.image, c.seeds, lab0, adj_r, per_class=pc)
+    assert np.array_equal(o2["label"], O.ref_oift(c.image, c.seeds, lab0, adj_r, per_class=pc))
+
+
+def _edges(shape, adj):
+    zs, ys, xs = shape
+    A = O.spheric(adj)
+    half = (len(A) - 1) // 2
+    out = []
+    for s in range(half):
+        dx, dy, dz = A[half + 1 + s]
+        for z in range(zs):
+            for y in range(ys):
+                for x in range(xs):
+                    qx, qy, qz = x + dx, y + dy, z + dz
+                    if 0 <= qx < xs and 0 <= qy < ys and 0 <= qz < zs:
+                        out.append((s, x + y * xs + z * xs * ys, qx + qy * xs + qz * xs * ys))
+    return out
+
+
+def _kruskal(n, edges_w, seeds, seed_labels):
+    """Union-find over edges sorted ascending; a component with a seed is frozen.
+    Returns (cost, label): cost = weight at which a voxel's component first touched a
+    seeded component (= minimax distance to the seed set), label = that seed's label
+    (well defined when the weights are distinct)."""
+    parent = list(range(n))
+    lab = [-1] * n
+    cost = [2**31 - 1] * n
+    members = {i: [i] for i in range(n)}
+    for s, l in zip(seeds, seed_labels):
+        lab[s] = int(l)
+        cost[s] = 0
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+    for w, p, q in sorted(edges_w):
+        a, b = find(p), find(q)
+        if a == b:
+            continue
+        la, lb = lab[a], lab[b]
+        if la >= 0 and lb >= 0:
+            continue
+        if la < 0 and lb < 0:
+            parent[b] = a
+            members[a] += members.pop(b)
+            continue
+        if la < 0:
+            a, b, la, lb = b, a, lb, la
+        for v in members[b]:
+            cost[v] = w
+            lab[v] = la
+        parent[b] = a
+        members[a] += members.pop(b)
+    return np.array(cost), np.array(lab)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_per0_costs_equal_kruskal_minimax(seed):
+    c = P.small_case((6, 7, 9), seed, per=0.0, top=60, noise=200.0)
+    I = c.image.astype(np.int64).ravel()
+    E = [(abs(int(I[p]) - int(I[q])), p, q) for _, p, q in _edges(c.image.shape, 1.0)]
+    kc, _ = _kruskal(I.size, E, c.seeds, c.seed_labels)
+    o = O.oift_image(c.image, c.seeds, c.label_init(), 1.0, 0.0)
+    assert np.array_equal(o["cost"].ravel(), kc)
+
+
+@pytest.mark.parametrize("adj", [1.0, 1.75])
+def test_distinct_weights_labels_equal_kruskal_forest(adj):
+    """per = 0 with all-distinct symmetric weights: the Prim forest is the seeded MSF."""
+    rng = np.random.default_rng(7)
+    shape = (5, 6, 7)
+    E = _edges(shape, adj)
+    n = int(np.prod(shape))
+    half = (len(O.spheric(adj)) - 1) // 2
+    perm = rng.permutation(len(E)) + 1
+    wa = np.full((half,) + shape, 0xFFFFFFFF, np.uint32)
+    for (s, p, q), w in zip(E, perm):
+        wa.reshape(half, -1)[s, p] = w
+    seeds = np.array([0, n - 1, n // 2 + 3, 17], np.int64)
+    labels = np.array([0, 0, 1, 2], np.int32)
+    lab0 = np.full(n, -1, np.int32)
+    lab0[seeds] = labels
+    o = O.oift_edges(wa, wa, seeds, lab0.reshape(shape), adj)
+    kc, kl = _kruskal(n, [(int(w), p, q) for (s, p, q), w in zip(E, perm)], seeds, labels)
+    assert np.array_equal(o["cost"].ravel(), kc)
+    assert np.array_equal(o["label"].ravel(), kl)
+
+
+def test_constant_image_labels_follow_fifo_bfs():
+    shape = (5, 6, 7)
+    zs, ys, xs = shape
+    img = np.full(shape, 42, np.int32)
+    n = img.size
+    seeds = np.array([n - 1, 0, 2 + 3 * xs + 2 * xs * ys, 5], np.int64)
+    labels = np.array([0, 1, 2, 0], np.int32)
+    lab = np.full(n, -1, np.int32)
+    lab[seeds] = labels
+    A = O.spheric(1.0)
+    bfs = lab.copy()
+    dq = deque(int(s) for s in seeds)
+    while dq:
+        p = dq.popleft()
+        x, y, z = p % xs, (p // xs) % ys, p // (xs * ys)
+        for dx, dy, dz in A[1:]:
+            qx, qy, qz = x + dx, y + dy, z + dz
+            if 0 <= qx < xs and 0 <= qy < ys and 0 <= qz < zs:
+                q = qx + qy * xs + qz * xs * ys
+                if bfs[q] == -1:
+                    bfs[q] = bfs[p]
+                    dq.append(q)
+    o = O.oift_image(img, seeds, lab.reshape(shape), 1.0, 50.0)
+    assert np.array_equal(o["label"].ravel(), bfs)
+    assert (o["cost"] == 0).all()
+
+
+def check_forest(I, shape, adj, per, seeds, label, cost, pred, per_class=None):
+    """Invariants any optimum-path forest of OIFT_Multi satisfies."""
+    zs, ys, xs = shape
+    A = O.spheric(adj)
+    lab = label.ravel()
+    C = cost.ravel()
+    pr = pred.ravel()
+    I = I.ravel().astype(np.int64)
+    assert (C[seeds] == 0).all()
+    n = I.size
+
+    def w(p, q):
+        if per_class is None:
+            pp = -per if lab[p] == 0 else per
+        else:
+            pp = per_class[lab[p]]
+        d = abs(int(I[p]) - int(I[q]))
+        if I[p] > I[q]:
+            return int(d * (1.0 + float(np.float32(pp)) / 100.0))
+        if I[p] < I[q]:
+            return int(d * (1.0 - float(np.float32(pp)) / 100.0))
+        return d
+    seedset = set(int(s) for s in seeds)
+    for q in range(n):
+        x, y, z = q % xs, (q // xs) % ys, q // (xs * ys)
+        if q not in seedset:
+            i = int(pr[q])
+            assert i > 0
+            dx, dy, dz = A[i]
+            p = (x - dx) + (y - dy) * xs + (z - dz) * xs * ys
+            assert lab[p] == lab[q]
+            assert C[q] == max(C[p], w(p, q))
+            assert C[p] <= C[q]
+        for dx, dy, dz in A[1:]:
+            qx, qy, qz = x + dx, y + dy, z + dz
+            if 0 <= qx < xs and 0 <= qy < ys and 0 <= qz < zs:
+                r = qx + qy * xs + qz * xs * ys
+                assert C[r] <= max(C[q], w(q, r))  # Bellman condition along every arc
+    # acyclic: following preds reaches a seed within n steps
+    for q in range(0, n, max(1, n // 50)):
+        v, k = q, 0
+        while v not in seedset:
+            x, y, z = v % xs, (v // xs) % ys, v // (xs * ys)
+            dx, dy, dz = A[int(pr[v])]
+            v = (x - dx) + (y - dy) * xs + (z - dz) * xs * ys
+            k += 1
+            assert k <= n
+
+
+@pytest.mark.parametrize("seed,adj,per", [(0, 1.0, 50.0), (1, 1.75, 50.0), (2, 1.0, -40.0), (3, 1.0, 100.0)])
+def test_oracle_forest_invariants(seed, adj, per):
+    c = P.small_case((7, 8, 9), seed, per=per, adj_r=adj, top=255)
+    o = O.oift_image(c.image, c.seeds, c.label_init(), adj, per)
+    check_forest(c.image, c.image.shape, adj, per, c.seeds, o["label"], o["cost"], o["pred"])
+
+
+@pytest.mark.parametrize("seed,adj,per", [(0, 1.0, 50.0), (5, 1.75, 50.0), (6, 1.0, -25.0), (7, 1.0, 37.5)])
+def test_two_priority_queues_agree(seed, adj, per):
+    c = P.small_case((11, 12, 13), seed, per=per, adj_r=adj, top=4095)
+    o = O.oift_image(c.image, c.seeds, c.label_init(), adj, per)
+    wa, wb = O.edge_weights(c.image, adj, per)
+    e = O.oift_edges(wa, wb, c.seeds, c.label_init(), adj)
+    for k in ("label", "cost"):
+        assert np.array_equal(o[k], e[k])
+
+
+def test_plain_mistake_is_caught():
+    """A planted mistake (background polarity sign flipped) must fail the golden pin."""
+    case = [c for c in golden_cases() if c["name"] == "ell6_1"][0]
+    good = O.oift_image(case["image"], case["seeds"], case["label0"], 1.0, case["per"])
+    bad = O.oift_image(case["image"], case["seeds"], case["label0"], 1.0, per_class=np.array([50.0, 50.0], np.float32))
+    assert np.array_equal(good["label"], case["oift_label"])
+    assert not np.array_equal(bad["label"], case["oift_label"])
