@@ -1,0 +1,140 @@
+"""Benchmark: OIFT (f_max) + 50-iteration fuzzy relaxation on a 256^3 int16 volume, 6-adjacency.
+
+One step = one pass of the hot path over one synthetic volume already resident
+in HBM: oriented weights + exact OIFT + 50 FP32 relaxation iterations + final
+labels (roift_pipeline_dev), preceded by restoring the seed-only label map.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--no-cpu]
+
+Prints one JSON line.  The CPU baseline runs the reference's own
+OIFT_Multi + ORelax_1_Multi (compiled from its sources into oracle/_ref) on one
+core over a bounded sample, or the oracle restatement when that build is absent.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per voxel (int16 image, uint16 weights, FP32 single-channel memberships):
+ALG = dict(weights=2 + 3 * 2 * 2, oift=12 + 4 + 4 + 4, relax_iter=2 + 4 + 4, labels=4)
+
+
+def cpu_baseline(size=128, runs=3):
+    import oracle as O
+    from roift_amd import phantom as P
+    c = P.headline_case(size)
+    lab0 = c.label_init()
+    kind = "reference" if O.ref_lib() is not None else "port"
+    times = []
+    for _ in range(runs):
+        if kind == "reference":
+            _, t1, t2 = O.ref_pipeline(c.image, c.seeds, lab0, c.niter, 1.0, c.per)
+            times.append(t1 + t2)
+        else:
+            t0 = time.perf_counter()
+            o = O.oift_image(c.image, c.seeds, lab0, 1.0, c.per)
+            O.relax(c.image, c.seeds, o["label"], c.niter, 1.0, c.per)
+            times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"value": round(c.image.size / t / 1e6, 4), "unit": "Mvoxels/s", "cores": 1, "kind": kind,
+            "sample": f"{size}^3 headline phantom (same generator), OIFT_Multi + 50-iteration ORelax_1_Multi, "
+                      f"median of {runs} runs of {t:.2f} s, 1 of {os.cpu_count()} host cores, "
+                      f"{'reference gft sources -O3' if kind == 'reference' else 'oracle restatement -O3'}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "volumes shard across GPUs as described in DESIGN.md but no multi-GPU path is built.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import roift_amd as R
+    from roift_amd import phantom as P
+
+    case = P.headline_case(args.size)
+    dev = torch.device("cuda:0")
+    ctx = R.Context(0)
+    img = torch.from_numpy(case.image.astype(np.int16)).to(dev)
+    seeds = torch.from_numpy(case.seeds.astype(np.int32)).to(dev)
+    lab0 = torch.from_numpy(case.label_init()).to(dev)
+    lab = torch.empty_like(lab0)
+    cost = torch.empty_like(lab0)
+    memb = torch.empty((2,) + tuple(img.shape), dtype=torch.float32, device=dev)
+    n = img.numel()
+
+    def step():
+        lab.copy_(lab0)
+        ctx.pipeline(img, seeds, lab, case.niter, 1.0, case.per, cost=cost, memb=memb)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stage = dict(weights_ms=[], oift_ms=[], relax_ms=[], oift_steps=0, flood_rounds=0)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        t = ctx.timings()
+        for k in ("weights_ms", "oift_ms", "relax_ms"):
+            stage[k].append(t[k])
+        stage["oift_steps"], stage["flood_rounds"] = int(t["oift_steps"]), int(t["flood_rounds"])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = dt / args.steps * 1e3
+    mvox = n * args.steps / dt / 1e6
+
+    # Roofline of the relaxation stencil (HBM-bound): algorithmic bytes per launch over the
+    # average launch time, from the stage's HIP events on the library's stream.
+    relax_ms = float(np.mean(stage["relax_ms"]))
+    relax_launch_s = relax_ms / 1e3 / case.niter
+    relax_gbs = ALG["relax_iter"] * n / relax_launch_s / 1e9
+    pipe_bytes = (ALG["weights"] + ALG["oift"] + ALG["relax_iter"] * case.niter + ALG["labels"]) * n
+    res = {
+        "metric": "Mvoxels/s OIFT+relax at 256^3 6-adj (and % HBM roofline); batched vs plain",
+        "value": round(mvox, 3),
+        "unit": "Mvoxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16/u64 int (OIFT) + f32 (relax)",
+        "data": "synthetic",
+        "config": {"workload": case.meta["workload"], "shape": list(img.shape), "image": "int16",
+                   "adjacency": 6, "alpha": 0.5, "relax_iters": case.niter, "seeds": int(seeds.numel())},
+        "stages_ms": {k: round(float(np.mean(v)), 3) for k, v in stage.items() if isinstance(v, list)},
+        "oift_level_steps": stage["oift_steps"],
+        "oift_flood_rounds": stage["flood_rounds"],
+        "pipeline_roofline": {"bound": "hbm", "algorithmic_bytes_per_voxel": pipe_bytes // n,
+                              "achieved": round(pipe_bytes / (dt / args.steps) / 1e9, 2),
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(pipe_bytes / (dt / args.steps) / 1e9 / HBM_PEAK_GBS, 5)},
+        "roofline": {"kernel": "k_relax2 (relaxation iteration)", "bound": "hbm", "achieved": round(relax_gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(relax_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline()
+        res["speedup_vs_cpu"] = round(res["value"] / res["cpu_baseline"]["value"], 1)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

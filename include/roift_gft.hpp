@@ -1,0 +1,61 @@
+// roift_gft.hpp -- link-level drop-in for the four gft entry points on the hot path.
+//
+// libroift_gft.so defines, with the same qualified names, signatures and types
+// as the reference's gft library, the functions its callers use on this path:
+//   gft::ift::OIFT_Multi              (gft_ift.h:108-112)
+//   gft::ift::OIFT_Multi_PerClass     (gft_ift.h:117-122)
+//   gft::ift::ORelax_1_Multi          (gft_ift.h:451-456)
+//   gft::ift::ORelax_1_Multi_PerClass (gft_ift.h:459-465)
+// Each copies the scene to the GPU, runs libroift.so (include/roift.h) and writes
+// the label scene back in place, exactly as the originals modify `label`.
+//
+// The record layouts below must match gft's (gft_scene32.h:20-27,
+// gft_adjrel3.h:10-23) because callers allocate these records with gft's own
+// constructors; only the fields this path reads are used (data, sizes, d, n).
+// A program that includes the reference's gft headers does not include this
+// file: it links -lroift_gft ahead of -lgft.
+#ifndef ROIFT_GFT_HPP
+#define ROIFT_GFT_HPP
+
+namespace gft {
+
+namespace AdjRel3 {
+union sDisplacement3 {
+    int data[4];
+    struct {
+        int x, y, z;
+    } axis;
+} __attribute__((aligned(16)));
+
+struct sAdjRel3 {
+    sDisplacement3 *d;
+    int n;
+};
+}  // namespace AdjRel3
+
+namespace Scene32 {
+struct sScene32 {
+    int *data;
+    int ***array;
+    int xsize, ysize, zsize;
+    float dx, dy, dz;
+    int maxval, minval, n;
+    void *nii_hdr;
+};
+}  // namespace Scene32
+
+typedef AdjRel3::sAdjRel3 sAdjRel3;
+typedef Scene32::sScene32 sScene32;
+
+namespace ift {
+void OIFT_Multi(sAdjRel3 *A, sScene32 *scn, float per, int *S, sScene32 *label);
+void OIFT_Multi_PerClass(sAdjRel3 *A, sScene32 *scn, const float *per_class, int max_label, int *S,
+                         sScene32 *label);
+void ORelax_1_Multi(sAdjRel3 *A, sScene32 *scn, float per, int *S, sScene32 *label, int ntimes);
+void ORelax_1_Multi_PerClass(sAdjRel3 *A, sScene32 *scn, const float *per_class, int max_label, int *S,
+                             sScene32 *label, int ntimes);
+}  // namespace ift
+
+}  // namespace gft
+
+#endif

@@ -1,0 +1,108 @@
+// gft_dropin.cpp -- gft-signature entry points over the C-ABI (libroift_gft.so).
+//
+// Same behaviour contract as the reference functions they replace
+// (gft_ift.cpp:911-1083, 8655-9187): `label` is modified in place; a NULL S
+// seeds every labelled voxel in raster order (gft_ift.cpp:944-948); where the
+// reference would call gft::Error (exit(-1)), this prints the library's error
+// and exits with the same status, since these void signatures cannot report it.
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+#include <vector>
+
+#include "../../include/roift.h"
+#include "../../include/roift_gft.hpp"
+
+namespace {
+
+roift_ctx *context()
+{
+    static roift_ctx *ctx = nullptr;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        if (roift_create(&ctx, 0) != ROIFT_OK) {
+            std::fprintf(stderr, "roift: no HIP device for the gft drop-in\n");
+            std::exit(-1);
+        }
+    });
+    return ctx;
+}
+
+float radius_of(const gft::sAdjRel3 *A)
+{
+    // Spheric(1.0) has 7 displacements, Spheric(1.75) has 27 (gft_adjrel3.cpp:47-85)
+    if (A->n == 7) return 1.0f;
+    if (A->n == 27) return 1.75f;
+    std::fprintf(stderr, "roift: adjacency with %d displacements is not supported\n", A->n);
+    std::exit(-1);
+}
+
+std::vector<int> seeds_of(const int *S, const gft::sScene32 *label)
+{
+    std::vector<int> s;
+    if (S) {
+        s.assign(S + 1, S + 1 + S[0]);
+    } else {
+        for (int p = 0; p < label->n; p++)
+            if (label->data[p] != -1) s.push_back(p);
+    }
+    return s;
+}
+
+void check(int rc)
+{
+    if (rc != ROIFT_OK) {
+        std::fprintf(stderr, "roift error %d: %s\n", rc, roift_last_error(context()));
+        std::exit(-1);
+    }
+}
+
+roift_problem problem(const gft::sAdjRel3 *A, const gft::sScene32 *scn, int use_pc, float per, const float *pc,
+                      int max_label)
+{
+    return roift_problem{scn->xsize, scn->ysize, scn->zsize, radius_of(A), use_pc, per, pc, max_label};
+}
+
+}  // namespace
+
+namespace gft {
+namespace ift {
+
+__attribute__((visibility("default"))) void OIFT_Multi(sAdjRel3 *A, sScene32 *scn, float per, int *S, sScene32 *label)
+{
+    const std::vector<int> s = seeds_of(S, label);
+    const roift_problem pb = problem(A, scn, 0, per, nullptr, -1);
+    check(roift_oift_fmax(context(), &pb, scn->data, s.data(), (int)s.size(), label->data, nullptr));
+}
+
+__attribute__((visibility("default"))) void OIFT_Multi_PerClass(sAdjRel3 *A, sScene32 *scn, const float *per_class,
+                                                                int max_label, int *S, sScene32 *label)
+{
+    const std::vector<int> s = seeds_of(S, label);
+    const roift_problem pb = problem(A, scn, 1, 0.0f, per_class, max_label);
+    check(roift_oift_fmax(context(), &pb, scn->data, s.data(), (int)s.size(), label->data, nullptr));
+}
+
+__attribute__((visibility("default"))) void ORelax_1_Multi(sAdjRel3 *A, sScene32 *scn, float per, int *S,
+                                                           sScene32 *label, int ntimes)
+{
+    if (S == nullptr || S[0] <= 0) {
+        std::fprintf(stderr, "roift: ORelax_1 (the empty-seed fallback) is not implemented on the GPU\n");
+        std::exit(-1);
+    }
+    const std::vector<int> s = seeds_of(S, label);
+    const roift_problem pb = problem(A, scn, 0, per, nullptr, -1);
+    check(roift_relax(context(), &pb, scn->data, s.data(), (int)s.size(), label->data, ntimes, nullptr));
+}
+
+__attribute__((visibility("default"))) void ORelax_1_Multi_PerClass(sAdjRel3 *A, sScene32 *scn, const float *per_class,
+                                                                    int max_label, int *S, sScene32 *label, int ntimes)
+{
+    if (S == nullptr || S[0] <= 0) return;  // gft_ift.cpp:8942-8943
+    const std::vector<int> s = seeds_of(S, label);
+    const roift_problem pb = problem(A, scn, 1, 0.0f, per_class, max_label);
+    check(roift_relax(context(), &pb, scn->data, s.data(), (int)s.size(), label->data, ntimes, nullptr));
+}
+
+}  // namespace ift
+}  // namespace gft

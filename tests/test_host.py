@@ -31,6 +31,33 @@ def test_library_exports_every_declared_symbol():
     assert set(R.EXPORTS) == set(names)
 
 
+GFT_SIGNATURES = {
+    "gft::ift::OIFT_Multi(gft::AdjRel3::sAdjRel3*, gft::Scene32::sScene32*, float, int*, gft::Scene32::sScene32*)",
+    "gft::ift::OIFT_Multi_PerClass(gft::AdjRel3::sAdjRel3*, gft::Scene32::sScene32*, float const*, int, int*, "
+    "gft::Scene32::sScene32*)",
+    "gft::ift::ORelax_1_Multi(gft::AdjRel3::sAdjRel3*, gft::Scene32::sScene32*, float, int*, gft::Scene32::sScene32*, "
+    "int)",
+    "gft::ift::ORelax_1_Multi_PerClass(gft::AdjRel3::sAdjRel3*, gft::Scene32::sScene32*, float const*, int, int*, "
+    "gft::Scene32::sScene32*, int)",
+}
+
+
+def _defined(path, dynamic=True):
+    import subprocess
+    out = subprocess.run(["nm", "-C"] + (["-D"] if dynamic else []) + [path], capture_output=True, text=True).stdout
+    return {line.split(" T ", 1)[1] for line in out.splitlines() if " T " in line}
+
+
+def test_gft_dropin_exports_reference_signatures():
+    """libroift_gft.so defines the four hot-path gft functions with the reference's mangled
+    signatures, so a caller links it ahead of -lgft unchanged."""
+    syms = _defined(os.path.join(ROOT, "roift_amd", "libroift_gft.so"))
+    assert GFT_SIGNATURES <= syms
+    ref_obj = os.path.join(ROOT, "oracle", "_ref", "obj", "gft_ift.cpp.o")
+    if os.path.exists(ref_obj):
+        assert GFT_SIGNATURES <= _defined(ref_obj, dynamic=False)
+
+
 def test_create_without_device_fails_cleanly():
     import torch
     if torch.cuda.is_available():
