@@ -34,9 +34,14 @@ class Case:
 
 
 def _face_seeds(shape) -> np.ndarray:
+    """Voxels on the volume's faces; a dimension of size 1 has no faces (a 2-D image's
+    border is its four edges -- seeding z == 0 == zsize-1 would seed every pixel)."""
     zs, ys, xs = shape
     z, y, x = np.meshgrid(np.arange(zs), np.arange(ys), np.arange(xs), indexing="ij")
-    face = (x == 0) | (x == xs - 1) | (y == 0) | (y == ys - 1) | (z == 0) | (z == zs - 1)
+    face = np.zeros(shape, bool)
+    for coord, size in ((x, xs), (y, ys), (z, zs)):
+        if size > 1:
+            face |= (coord == 0) | (coord == size - 1)
     return np.flatnonzero(face.ravel())
 
 
@@ -71,9 +76,9 @@ def cube_seeds(shape, center_voxel, half: int) -> np.ndarray:
     zs, ys, xs = shape
     cx, cy, cz = center_voxel
     idx = [x + y * xs + z * xs * ys
-           for z in range(cz - half, cz + half + 1)
-           for y in range(cy - half, cy + half + 1)
-           for x in range(cx - half, cx + half + 1)]
+           for z in range(max(0, cz - half), min(zs, cz + half + 1))
+           for y in range(max(0, cy - half), min(ys, cy + half + 1))
+           for x in range(max(0, cx - half), min(xs, cx + half + 1))]
     return np.array(idx, np.int64)
 
 

@@ -98,7 +98,7 @@ def test_oift_frozen_labels_and_unreached_voxels(ctx):
     """Preset non-seed labels are never popped and block propagation; voxels walled
     off by them keep NIL and cost INT32_MAX, as in the reference (gft_ift.cpp:930-943)."""
     c = P.small_case((10, 12, 14), 12, per=50.0, top=255)
-    lab0 = c.label_init().copy()
+    lab0 = np.full(c.image.shape, -1, np.int32)
     lab0[:, :, 3] = 2            # a frozen wall of label 2 (not in S) at x = 3
     keep = (c.seeds % 14) > 3    # no seeds left of the wall: x < 3 stays unreached
     seeds, labels = c.seeds[keep], c.seed_labels[keep]
