@@ -120,6 +120,8 @@ def main():
         "config": {"workload": case.meta["workload"], "shape": list(img.shape), "image": "int16",
                    "adjacency": 6, "alpha": 0.5, "relax_iters": case.niter, "seeds": int(seeds.numel())},
         "stages_ms": {k: round(float(np.mean(v)), 3) for k, v in stage.items() if isinstance(v, list)},
+        "oift_phase_ms": {k: v[0] for k, v in ctx.oift_profile().items()},
+        "oift_phase_count": {k: v[1] for k, v in ctx.oift_profile().items()},
         "oift_level_steps": stage["oift_steps"],
         "oift_flood_rounds": stage["flood_rounds"],
         "pipeline_roofline": {"bound": "hbm", "algorithmic_bytes_per_voxel": pipe_bytes // n,

@@ -60,6 +60,20 @@ ROIFT_API int roift_set_stream(roift_ctx *ctx, void *hip_stream);
 /* Stage timings (device events) of the last call, in milliseconds:
    [0] weights, [1] oift, [2] relax, [3] labels; [4] OIFT level steps, [5] flood rounds. */
 ROIFT_API int roift_last_timings(const roift_ctx *ctx, double *out6);
+/* Wall-clock breakdown of the last OIFT's persistent kernel, measured on the
+   device between its grid barriers: out[0..8] milliseconds spent in the phases
+   level-scan, collect, key-collect, key-rank, flood-seed, flood-round,
+   r_safe, claim, offers; out[9..17] how many times each phase ran; with
+   n >= 162, out[18 + 16*phase + b] splits each phase's time by the step's
+   entry count (bucket b = ceil(log2(entries)), capped at 15).  n >= 18.      */
+ROIFT_API int roift_oift_profile(const roift_ctx *ctx, double *out, int n);
+
+/* Tuning knobs (the autotuner in roift_amd/tune.py sets them):
+   ROIFT_PARAM_OIFT_BLOCKS    workgroups of the persistent OIFT kernel (0 = one per CU)
+   ROIFT_PARAM_FLOOD_BUDGET   LDS batches a workgroup relaxes per flood round before spilling
+   ROIFT_PARAM_RELAX_DEPTH    relaxation iterations fused per pass (temporal-block depth)   */
+enum roift_param { ROIFT_PARAM_OIFT_BLOCKS = 1, ROIFT_PARAM_FLOOD_BUDGET = 2, ROIFT_PARAM_RELAX_DEPTH = 3 };
+ROIFT_API int roift_set_param(roift_ctx *ctx, int param, int value);
 
 /* ---- oriented arc-weight builder --------------------------------------------- */
 /* Replaces the weight computed inline in OIFT_Multi (gft_ift.cpp:965-973).

@@ -24,6 +24,7 @@ NIL = -1
 
 EXPORTS = (
     "roift_create", "roift_destroy", "roift_last_error", "roift_set_stream", "roift_last_timings",
+    "roift_oift_profile", "roift_set_param",
     "roift_weights_dev", "roift_oift_fmax_dev", "roift_oift_edges_dev", "roift_relax_dev",
     "roift_pipeline_dev", "roift_oift_fmax", "roift_relax", "oift_fmax_gpu_run",
 )
@@ -60,6 +61,8 @@ def lib():
         L.roift_last_error.restype = C.c_char_p
         L.roift_set_stream.argtypes = [vp, vp]
         L.roift_last_timings.argtypes = [vp, dp]
+        L.roift_oift_profile.argtypes = [vp, dp, ip]
+        L.roift_set_param.argtypes = [vp, ip, ip]
         L.roift_weights_dev.argtypes = [vp, pp, vp, ip, vp, vp, ip, i32p, i32p]
         L.roift_oift_fmax_dev.argtypes = [vp, pp, vp, ip, vp, ip, vp, vp]
         L.roift_oift_edges_dev.argtypes = [vp, pp, vp, vp, ip, i8p, vp, ip, vp, vp]
@@ -128,6 +131,24 @@ class Context:
         self._check(self._L.roift_last_timings(self._h, out))
         keys = ("weights_ms", "oift_ms", "relax_ms", "labels_ms", "oift_steps", "flood_rounds")
         return dict(zip(keys, list(out)))
+
+    PARAMS = {"oift_blocks": 1, "flood_budget": 2, "relax_depth": 3}
+
+    def set_param(self, name, value):
+        self._check(self._L.roift_set_param(self._h, self.PARAMS[name], int(value)))
+
+    PHASES = ("level_scan", "collect", "key_collect", "key_rank", "flood_seed", "flood_round", "rsafe",
+              "claim", "offers")
+
+    def oift_profile(self, by_size=False):
+        """Per-phase device wall-clock ms and counts of the last OIFT (and, with
+        by_size, each phase's ms split by ceil(log2(entries of the step)))."""
+        out = (C.c_double * 162)()
+        self._check(self._L.roift_oift_profile(self._h, out, 162))
+        res = {ph: (round(out[i], 3), int(out[9 + i])) for i, ph in enumerate(self.PHASES)}
+        if by_size:
+            res = {ph: [round(out[18 + 16 * i + b], 2) for b in range(16)] for i, ph in enumerate(self.PHASES)}
+        return res
 
     # ---- device (torch) forms -------------------------------------------------------
     def weights(self, image, adj_r=1.0, per=0.0, wdtype="u16"):

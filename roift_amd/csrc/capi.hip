@@ -197,6 +197,24 @@ ROIFT_API int roift_last_timings(const roift_ctx *c, double *out6)
     return ROIFT_OK;
 }
 
+ROIFT_API int roift_set_param(roift_ctx *c, int param, int value)
+{
+    if (!c || value < 0) return ROIFT_EINVAL;
+    switch (param) {
+    case ROIFT_PARAM_OIFT_BLOCKS: c->tune_blocks = value; return ROIFT_OK;
+    case ROIFT_PARAM_FLOOD_BUDGET: c->tune_budget = value; return ROIFT_OK;
+    case ROIFT_PARAM_RELAX_DEPTH: c->tune_relax_depth = value; return ROIFT_OK;
+    default: return ROIFT_EINVAL;
+    }
+}
+
+ROIFT_API int roift_oift_profile(const roift_ctx *c, double *out, int n)
+{
+    if (!c || !out || n < 18) return ROIFT_EINVAL;
+    for (int i = 0; i < n && i < 162; i++) out[i] = c->prof[i];
+    return ROIFT_OK;
+}
+
 ROIFT_API int roift_weights_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
                                 void *d_wa, void *d_wb, int weight_dtype, int32_t *h_maxI, int32_t *h_minI)
 {

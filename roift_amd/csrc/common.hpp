@@ -42,6 +42,41 @@ struct Dims {
     long long n;
 };
 
+// Division by an invariant d for 0 <= n < 2^31 with one mul-hi and a shift
+// (Granlund & Montgomery, "Division by invariant integers using multiplication", 1994).
+struct FastDiv {
+    uint32_t d = 1, m = 1;
+    int s = 0;
+    FastDiv() = default;
+    explicit FastDiv(uint32_t dv) : d(dv)
+    {
+        while ((1ull << s) < dv) s++;
+        m = (uint32_t)((((1ull << 32) * ((1ull << s) - dv)) / dv) + 1);
+    }
+    __host__ __device__ inline uint32_t div(uint32_t n) const
+    {
+#ifdef __HIP_DEVICE_COMPILE__
+        return (__umulhi(n, m) + n) >> s;
+#else
+        return (uint32_t)((((uint64_t)n * m) >> 32) + n) >> s;
+#endif
+    }
+};
+
+// Voxel coordinates of a linear index (volumes below 2^31 voxels).
+struct Coords {
+    FastDiv fx, fy;
+    int xs, ys, zs;
+    explicit Coords(const Dims &d) : fx((uint32_t)d.xs), fy((uint32_t)d.ys), xs(d.xs), ys(d.ys), zs(d.zs) {}
+    __device__ inline void of(uint32_t p, int &x, int &y, int &z) const
+    {
+        const uint32_t t = fx.div(p);
+        x = (int)(p - t * (uint32_t)xs);
+        z = (int)fy.div(t);
+        y = (int)(t - (uint32_t)z * (uint32_t)ys);
+    }
+};
+
 __host__ __device__ inline bool in_volume(const Dims &d, int x, int y, int z)
 {
     return (unsigned)x < (unsigned)d.xs && (unsigned)y < (unsigned)d.ys && (unsigned)z < (unsigned)d.zs;
@@ -81,6 +116,24 @@ __device__ inline uint32_t wave_append_index(uint32_t *counter, bool want)
     if (!want) return INF32;
     const unsigned long long below = mask & ((1ull << lane) - 1ull);
     return base + (uint32_t)__popcll(below);
+}
+
+// Wave-aggregated append of `cnt` items per lane: returns this lane's first index in the
+// list (one atomicAdd per wavefront).  Every lane of the wavefront must call it.
+__device__ inline uint32_t wave_reserve(uint32_t *counter, uint32_t cnt)
+{
+    const int lane = __lane_id();
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_up((int)incl, o);
+        if (lane >= o) incl += v;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)incl, 63);
+    uint32_t base = 0;
+    if (lane == 0 && total) base = atomicAdd(counter, total);
+    base = (uint32_t)__shfl((int)base, 0);
+    return base + incl - cnt;
 }
 
 // Fill without hipMemsetAsync (kernels are safe inside captured graphs).
