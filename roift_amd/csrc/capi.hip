@@ -282,7 +282,7 @@ ROIFT_API int roift_relax_dev(roift_ctx *ctx, const roift_problem *pb, const voi
         const RelaxSpec rs = relax_spec_of(pb);
         ev_record(*ctx, 2);
         const WeightInfo wi = image_extrema(*ctx, d, d_image, image_dtype);
-        run_relax(*ctx, d, nadj, d_image, image_dtype, rs, d_seeds, n_seeds, d_label, niter, d_memb, h_nclasses, wi.maxI);
+        run_relax(*ctx, d, nadj, d_image, image_dtype, rs, d_seeds, n_seeds, d_label, niter, d_memb, h_nclasses, wi);
         ev_record(*ctx, 3);
         ctx->timings[2] = ev_ms(*ctx, 2, 3);
         return ROIFT_OK;
@@ -303,7 +303,7 @@ ROIFT_API int roift_pipeline_dev(roift_ctx *ctx, const roift_problem *pb, const 
         ev_record(*ctx, 1);
         run_oift(*ctx, d, nadj, w.wa, w.wb, w.dtype, po.lc, d_seeds, n_seeds, d_label, d_cost);
         ev_record(*ctx, 2);
-        run_relax(*ctx, d, nadj, d_image, image_dtype, rs, d_seeds, n_seeds, d_label, niter, d_memb, nullptr, w.info.maxI);
+        run_relax(*ctx, d, nadj, d_image, image_dtype, rs, d_seeds, n_seeds, d_label, niter, d_memb, nullptr, w.info);
         ev_record(*ctx, 3);
         ctx->timings[0] = ev_ms(*ctx, 0, 1);
         ctx->timings[1] = ev_ms(*ctx, 1, 2);

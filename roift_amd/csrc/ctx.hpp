@@ -86,6 +86,6 @@ struct RelaxSpec {
 
 int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, const RelaxSpec &rs,
               const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter, float *d_memb, int32_t *nclasses_out,
-              int32_t maxI);
+              const WeightInfo &wi);
 
 }  // namespace roift

@@ -32,6 +32,7 @@ struct RelaxArgs {
     float per;          // OIFT_Multi form
     float pcm[NC_MAX];  // per-class form, mapped to compact classes
     float Wmax;
+    float tA, tB;       // s/100 = tA * p_bg(q) + tB  (z-marching form)
     float Dpq[27];
     int use_pc;
     int nc;
@@ -215,6 +216,115 @@ __global__ void k_memb_out2(long long n, const float *__restrict__ f, int bg, fl
     }
 }
 
+// ---------------------------------------------------------------- z-marching form
+// 6-adjacency, two classes: each thread owns one (x, y) column of a 64x4 tile and
+// marches a chunk of z planes, keeping the z-1 / z / z+1 values in registers and
+// reading its x/y neighbours (L1/L2-resident) directly.  The image is int16 with the
+// seed flag in bit 15 (intensities are 0..32767); seeds keep their membership.
+constexpr int MX = 64, MY = 4, MZ = 32;
+
+// Weight of neighbour q in p's update, restated for fewer operations:
+//   |d| * (1 +- s/100) = |d| + d*t  with d = I_p - I_q and t = s/100 = tA*f_q + tB
+// (the polarity's sign is the sign of d).  Rounding differs from the reference's
+// FP32 order by a few ulp; results stay within 1e-5 of the FP64 oracle.
+__device__ inline float relax_w8(float d, float fq, const RelaxArgs &a)
+{
+    const float t = fmaf(a.tA, fq, a.tB);
+    float w = a.Wmax - fmaf(d, t, fabsf(d));
+    w = fmaxf(w, 0.0f);
+    w = w * w;
+    w = w * w;
+    return w * w;
+}
+
+__global__ __launch_bounds__(MX *MY) void k_relax2_march(const int16_t *__restrict__ img, RelaxArgs a,
+                                                          const float *__restrict__ fin, float *__restrict__ fout,
+                                                          int32_t *__restrict__ label_out)
+{
+    const int xs = a.d.xs, ys = a.d.ys, zs = a.d.zs;
+    const int x = blockIdx.x * MX + threadIdx.x, y = blockIdx.y * MY + threadIdx.y;
+    const int z0 = blockIdx.z * MZ, z1 = min(zs, z0 + MZ);
+    if (x >= xs || y >= ys || z0 >= zs) return;   // no block-level synchronisation below
+    const int xy = xs * ys;
+    int p = x + y * xs + z0 * xy;
+    const int lab_bg = a.cls_label[a.bg], lab_ob = a.cls_label[1 - a.bg];
+    bool hm = z0 > 0;
+    float Im = 0.0f, fm = 0.0f;
+    if (hm) {
+        Im = (float)(img[p - xy] & 0x7FFF);
+        fm = fin[p - xy];
+    }
+    int rc = img[p];
+    float fc = fin[p];
+    for (int z = z0; z < z1; z++, p += xy) {
+        const bool hp = z + 1 < zs;
+        int rn = 0;
+        float fn = 0.0f;
+        if (hp) {
+            rn = img[p + xy];
+            fn = fin[p + xy];
+        }
+        const float Ic = (float)(rc & 0x7FFF);
+        float v = fc;
+        if (!(rc & 0x8000)) {
+            // neighbours in Spheric order: -z, -y, -x, +x, +y, +z (the reference's summation order)
+            float sw = 0.0f, acc = 0.0f, w;
+            if (hm) {
+                w = relax_w8(Ic - Im, fm, a);
+                sw += w;
+                acc = fmaf(w, fm, acc);
+            }
+            if (y > 0) {
+                const float fq = fin[p - xs];
+                w = relax_w8(Ic - (float)(img[p - xs] & 0x7FFF), fq, a);
+                sw += w;
+                acc = fmaf(w, fq, acc);
+            }
+            if (x > 0) {
+                const float fq = fin[p - 1];
+                w = relax_w8(Ic - (float)(img[p - 1] & 0x7FFF), fq, a);
+                sw += w;
+                acc = fmaf(w, fq, acc);
+            }
+            if (x < xs - 1) {
+                const float fq = fin[p + 1];
+                w = relax_w8(Ic - (float)(img[p + 1] & 0x7FFF), fq, a);
+                sw += w;
+                acc = fmaf(w, fq, acc);
+            }
+            if (y < ys - 1) {
+                const float fq = fin[p + xs];
+                w = relax_w8(Ic - (float)(img[p + xs] & 0x7FFF), fq, a);
+                sw += w;
+                acc = fmaf(w, fq, acc);
+            }
+            if (hp) {
+                const float In = (float)(rn & 0x7FFF);
+                w = relax_w8(Ic - In, fn, a);
+                sw += w;
+                acc = fmaf(w, fn, acc);
+            }
+            v = (sw > 0.0f) ? acc / sw : fc;
+        }
+        fout[p] = v;
+        if (label_out) label_out[p] = ((1.0f - v) > v) ? lab_ob : lab_bg;
+        hm = true;
+        Im = Ic;
+        fm = fc;
+        rc = rn;
+        fc = fn;
+    }
+}
+
+// int16 image with the seed flag in bit 15, for the z-marching form
+template <typename IT>
+__global__ void k_pack_image(const IT *__restrict__ img, const uint8_t *__restrict__ seedcls, long long n,
+                             int16_t *__restrict__ out)
+{
+    for (long long p = gtid(); p < n; p += gstride())
+        out[p] = (int16_t)((int)img[p] | (seedcls[p] != NOSEED ? 0x8000 : 0));
+}
+
 template <int NADJ, typename IT>
 void launch_iter(roift_ctx &c, const void *img, const RelaxArgs &a, const float *fin, float *fout, const uint8_t *seedcls,
                  int32_t *label_out)
@@ -233,8 +343,9 @@ void launch_iter(roift_ctx &c, const void *img, const RelaxArgs &a, const float 
 
 int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, const RelaxSpec &rs,
               const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter, float *d_memb, int32_t *nclasses_out,
-              int32_t maxI)
+              const WeightInfo &wi)
 {
+    const int32_t maxI = wi.maxI;
     ROIFT_REQUIRE(nadj == 7 || nadj == 27, ROIFT_EUNSUPPORTED, "adjacency must be Spheric(1.0) or Spheric(1.75)");
     ROIFT_REQUIRE(n_seeds > 0, ROIFT_EINVAL, "relaxation needs seeds (the S==NULL form is ORelax_1)");
     ROIFT_REQUIRE(niter >= 0, ROIFT_EINVAL, "negative iteration count");
@@ -297,6 +408,15 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
                              (float)(az * az) * 1.0f * 1.0f) / 1.0f;
         }
     for (int k = 0; k < nc; k++) a.cls_label[k] = cls[k];
+    if (nc == 2) {
+        if (rs.use_per_class) {
+            a.tA = (float)(((double)a.pcm[bg] - (double)a.pcm[1 - bg]) / 100.0);
+            a.tB = (float)((double)a.pcm[1 - bg] / 100.0);
+        } else {
+            a.tA = (float)(2.0 * (double)rs.per / 100.0);
+            a.tB = (float)(-(double)rs.per / 100.0);
+        }
+    }
 
     int16_t *dl2c = buf<int16_t>(c, "relax.l2c", (size_t)maxl + 1);
     ROIFT_HIP_CHECK(hipMemcpyAsync(dl2c, l2c.data(), l2c.size() * sizeof(int16_t), hipMemcpyHostToDevice, st));
@@ -315,10 +435,25 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
         hipLaunchKernelGGL(k_labels_from_init, dim3(G), dim3(256), 0, st, n, d_label, dl2c, maxl, bg, cl, d_label);
     }
     float *fin = f0, *fout = f1;
+    const bool i16 = image_dtype == ROIFT_I16;
+    // z-marching form: 6-adjacency, two classes, intensities in [0, 32767] (bit 15 = seed)
+    const bool march = nadj == 7 && nc == 2 && wi.minI >= 0 && wi.maxI <= 0x7FFF && n < (1ll << 31) && niter > 0;
+    int16_t *pimg = nullptr;
+    if (march) {
+        pimg = buf<int16_t>(c, "relax.pimg", n);
+        if (i16)
+            hipLaunchKernelGGL(k_pack_image<int16_t>, dim3(G), dim3(256), 0, st, (const int16_t *)d_image, seedcls, n, pimg);
+        else
+            hipLaunchKernelGGL(k_pack_image<int32_t>, dim3(G), dim3(256), 0, st, (const int32_t *)d_image, seedcls, n, pimg);
+        ROIFT_HIP_CHECK(hipGetLastError());
+    }
+    const dim3 mgrid((d.xs + MX - 1) / MX, (d.ys + MY - 1) / MY, (d.zs + MZ - 1) / MZ);
     for (int it = 0; it < niter; it++) {
         int32_t *lo = (it == niter - 1) ? d_label : nullptr;
-        const bool i16 = image_dtype == ROIFT_I16;
-        if (nadj == 7) {
+        if (march) {
+            hipLaunchKernelGGL(k_relax2_march, mgrid, dim3(MX, MY), 0, st, pimg, a, fin, fout, lo);
+            ROIFT_HIP_CHECK(hipGetLastError());
+        } else if (nadj == 7) {
             if (i16) launch_iter<7, int16_t>(c, d_image, a, fin, fout, seedcls, lo);
             else launch_iter<7, int32_t>(c, d_image, a, fin, fout, seedcls, lo);
         } else {

@@ -14,53 +14,60 @@
 
 namespace roift {
 
+// Block shape of the weight builder: 64 x 4 (x, y) voxels, marching WZ planes in z.
+constexpr int WX = 64, WY = 4, WZ = 16;
+
 template <int NADJ, typename IT, typename WT>
-__global__ __launch_bounds__(256) void k_weights(const IT *__restrict__ img, Dims d, double fp, double fm,
-                                                 WT *__restrict__ wa, WT *__restrict__ wb, int *__restrict__ ext,
-                                                 uint32_t *__restrict__ overflow)
+__global__ __launch_bounds__(WX *WY) void k_weights(const IT *__restrict__ img, Dims d, double fp, double fm,
+                                                   WT *__restrict__ wa, WT *__restrict__ wb, int *__restrict__ ext,
+                                                   uint32_t *__restrict__ overflow)
 {
     using A = Adjacency<NADJ>;
     constexpr uint32_t WMAXV = sizeof(WT) == 2 ? 0xFFFFu : 0xFFFFFFFFu;
     const long long xy = (long long)d.xs * d.ys;
+    const int x = blockIdx.x * WX + threadIdx.x, y = blockIdx.y * WY + threadIdx.y;
+    const int z0 = blockIdx.z * WZ, z1 = min(d.zs, z0 + WZ);
     int lmax = INT32_MIN, lmin = INT32_MAX;
     uint32_t ovf = 0;
-    for (long long p = gtid(); p < d.n; p += gstride()) {
-        const int x = (int)(p % d.xs), y = (int)((p / d.xs) % d.ys), z = (int)(p / xy);
-        const int Ip = (int)img[p];
-        lmax = max(lmax, Ip);
-        lmin = min(lmin, Ip);
+    if (x < d.xs && y < d.ys) {
+        for (int z = z0; z < z1; z++) {
+            const long long p = x + (long long)y * d.xs + (long long)z * xy;
+            const int Ip = (int)img[p];
+            lmax = max(lmax, Ip);
+            lmin = min(lmin, Ip);
 #pragma unroll
-        for (int s = 0; s < A::half; s++) {
-            const int i = A::half + 1 + s;
-            const int qx = x + A::dx(i), qy = y + A::dy(i), qz = z + A::dz(i);
-            uint32_t a = WMAXV, b = WMAXV;
-            if (in_volume(d, qx, qy, qz)) {
-                const long long q = qx + (long long)qy * d.xs + (long long)qz * xy;
-                const int Iq = (int)img[q];
-                const int dd = abs(Ip - Iq);
-                // w_obj(p->q): per_pq=+per; w_obj(q->p): roles of p and q swapped
-                const int up = (int)((double)dd * fp), dn = (int)((double)dd * fm);
-                a = (uint32_t)(Ip > Iq ? up : (Ip < Iq ? dn : dd));
-                b = (uint32_t)(Iq > Ip ? up : (Iq < Ip ? dn : dd));
-                if (a >= WMAXV || b >= WMAXV) ovf = 1;
+            for (int s = 0; s < A::half; s++) {
+                const int i = A::half + 1 + s;
+                const int qx = x + A::dx(i), qy = y + A::dy(i), qz = z + A::dz(i);
+                uint32_t a = WMAXV, b = WMAXV;
+                if (in_volume(d, qx, qy, qz)) {
+                    const long long q = p + A::dx(i) + (long long)A::dy(i) * d.xs + (long long)A::dz(i) * xy;
+                    const int Iq = (int)img[q];
+                    const int dd = abs(Ip - Iq);
+                    // w_obj(p->q): per_pq=+per; w_obj(q->p): roles of p and q swapped
+                    const int up = (int)((double)dd * fp), dn = (int)((double)dd * fm);
+                    a = (uint32_t)(Ip > Iq ? up : (Ip < Iq ? dn : dd));
+                    b = (uint32_t)(Iq > Ip ? up : (Iq < Ip ? dn : dd));
+                    if (a >= WMAXV || b >= WMAXV) ovf = 1;
+                }
+                wa[s * d.n + p] = (WT)a;
+                wb[s * d.n + p] = (WT)b;
             }
-            wa[s * d.n + p] = (WT)a;
-            wb[s * d.n + p] = (WT)b;
         }
     }
     // block reduction of the extrema, then one atomic per block
-    __shared__ int smax[8], smin[8];
+    __shared__ int smax[WX * WY / 64], smin[WX * WY / 64];
     for (int o = 32; o > 0; o >>= 1) {
         lmax = max(lmax, __shfl_xor(lmax, o));
         lmin = min(lmin, __shfl_xor(lmin, o));
     }
-    const int wid = threadIdx.x / 64;
+    const int tid = threadIdx.x + threadIdx.y * WX, wid = tid / 64;
     if (__lane_id() == 0) { smax[wid] = lmax; smin[wid] = lmin; }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < (int)(blockDim.x / 64); w++) { lmax = max(lmax, smax[w]); lmin = min(lmin, smin[w]); }
-        atomicMax(&ext[0], lmax);
-        atomicMin(&ext[1], lmin);
+    if (tid == 0) {
+        for (int w = 1; w < WX * WY / 64; w++) { lmax = max(lmax, smax[w]); lmin = min(lmin, smin[w]); }
+        if (lmax != INT32_MIN) atomicMax(&ext[0], lmax);
+        if (lmin != INT32_MAX) atomicMin(&ext[1], lmin);
     }
     if (ovf) atomicOr(overflow, 1u);
 }
@@ -95,8 +102,11 @@ template <int NADJ, typename IT, typename WT>
 void launch_weights(roift_ctx &c, const Dims &d, const void *img, double fp, double fm, void *wa, void *wb, int *ext,
                     uint32_t *ovf)
 {
-    hipLaunchKernelGGL((k_weights<NADJ, IT, WT>), dim3(grid_for(d.n, 256, 1 << 16)), dim3(256), 0, c.stream,
-                       (const IT *)img, d, fp, fm, (WT *)wa, (WT *)wb, ext, ovf);
+    ROIFT_REQUIRE((d.ys + WY - 1) / WY <= 65535 && (d.zs + WZ - 1) / WZ <= 65535, ROIFT_EUNSUPPORTED,
+                  "volume too large for the weight builder's grid");
+    const dim3 grid((d.xs + WX - 1) / WX, (d.ys + WY - 1) / WY, (d.zs + WZ - 1) / WZ);
+    hipLaunchKernelGGL((k_weights<NADJ, IT, WT>), grid, dim3(WX, WY), 0, c.stream, (const IT *)img, d, fp, fm, (WT *)wa,
+                       (WT *)wb, ext, ovf);
     ROIFT_HIP_CHECK(hipGetLastError());
 }
 
