@@ -24,6 +24,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # algorithmic HBM bytes per voxel (int16 image, uint16 weights, FP32 single-channel memberships):
 ALG = dict(weights=2 + 3 * 2 * 2, oift=12 + 4 + 4 + 4, relax_iter=2 + 4 + 4, labels=4)
+# HBM bytes per voxel per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
+# workload, scaled by the same counters on a 1 GiB copy (profiles/pmc_traffic_headline256.txt).
+TRAFFIC = {"k_oift_persistent": 2704.9, "k_relax2_march": 19.74, "k_weights": 16.13}
 
 
 def cpu_baseline(size=128, runs=3):
@@ -56,6 +59,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe-copy", action="store_true", help="also run a 1 GiB device copy (profiler scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -86,6 +90,7 @@ def main():
         step()
     torch.cuda.synchronize()
     stage = dict(weights_ms=[], oift_ms=[], relax_ms=[], oift_steps=0, flood_rounds=0)
+    kern = dict(weights_ms=[], oift_kernel_ms=[], relax_loop_ms=[])
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
@@ -93,16 +98,28 @@ def main():
         for k in ("weights_ms", "oift_ms", "relax_ms"):
             stage[k].append(t[k])
         stage["oift_steps"], stage["flood_rounds"] = int(t["oift_steps"]), int(t["flood_rounds"])
+        kt = ctx.kernel_timings()
+        for k in kern:
+            kern[k].append(kt[k])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ms = dt / args.steps * 1e3
     mvox = n * args.steps / dt / 1e6
 
-    # Roofline of the relaxation stencil (HBM-bound): algorithmic bytes per launch over the
-    # average launch time, from the stage's HIP events on the library's stream.
-    relax_ms = float(np.mean(stage["relax_ms"]))
-    relax_launch_s = relax_ms / 1e3 / case.niter
-    relax_gbs = ALG["relax_iter"] * n / relax_launch_s / 1e9
+    # Rooflines from device events recorded on the library's stream around each kernel:
+    # algorithmic bytes per launch / average launch duration.
+    def roof(name, alg_per_voxel, launch_ms):
+        gbs = alg_per_voxel * n / (launch_ms / 1e3) / 1e9
+        out = {"kernel": name, "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(gbs / HBM_PEAK_GBS, 5), "launch_ms": round(launch_ms, 4),
+               "alg_bytes_per_launch": alg_per_voxel * n}
+        tr = TRAFFIC.get(name)
+        out["traffic"] = int(tr * n) if tr is not None else None
+        return out
+
+    oift_k = float(np.mean(kern["oift_kernel_ms"]))
+    relax_k = float(np.mean(kern["relax_loop_ms"])) / case.niter
+    weights_k = float(np.mean(kern["weights_ms"]))
     pipe_bytes = (ALG["weights"] + ALG["oift"] + ALG["relax_iter"] * case.niter + ALG["labels"]) * n
     res = {
         "metric": "Mvoxels/s OIFT+relax at 256^3 6-adj (and % HBM roofline); batched vs plain",
@@ -120,6 +137,8 @@ def main():
         "config": {"workload": case.meta["workload"], "shape": list(img.shape), "image": "int16",
                    "adjacency": 6, "alpha": 0.5, "relax_iters": case.niter, "seeds": int(seeds.numel())},
         "stages_ms": {k: round(float(np.mean(v)), 3) for k, v in stage.items() if isinstance(v, list)},
+        "kernel_ms": {"weights": round(weights_k, 4), "oift_persistent": round(oift_k, 3),
+                      "relax_iteration": round(relax_k, 4)},
         "oift_phase_ms": {k: v[0] for k, v in ctx.oift_profile().items()},
         "oift_phase_count": {k: v[1] for k, v in ctx.oift_profile().items()},
         "oift_level_steps": stage["oift_steps"],
@@ -128,10 +147,17 @@ def main():
                               "achieved": round(pipe_bytes / (dt / args.steps) / 1e9, 2),
                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(pipe_bytes / (dt / args.steps) / 1e9 / HBM_PEAK_GBS, 5)},
-        "roofline": {"kernel": "k_relax2 (relaxation iteration)", "bound": "hbm", "achieved": round(relax_gbs, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(relax_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+        # the dominant kernel (most of the step's time): the persistent OIFT kernel
+        "roofline": roof("k_oift_persistent", ALG["oift"], oift_k),
+        "other_rooflines": [roof("k_relax2_march", ALG["relax_iter"], relax_k),
+                            roof("k_weights", ALG["weights"], weights_k)],
     }
+    if args.probe_copy:
+        # a plain copy of known size, for scaling the profiler's FETCH_SIZE / WRITE_SIZE
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline()
         res["speedup_vs_cpu"] = round(res["value"] / res["cpu_baseline"]["value"], 1)

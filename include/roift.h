@@ -60,6 +60,11 @@ ROIFT_API int roift_set_stream(roift_ctx *ctx, void *hip_stream);
 /* Stage timings (device events) of the last call, in milliseconds:
    [0] weights, [1] oift, [2] relax, [3] labels; [4] OIFT level steps, [5] flood rounds. */
 ROIFT_API int roift_last_timings(const roift_ctx *ctx, double *out6);
+/* Device-event times of the last call's main kernels, in milliseconds:
+   [0] weight builder, [1] OIFT persistent kernel, [2] all relaxation iterations
+   (back-to-back launches), [3] number of relaxation launches.  n >= 4.
+   (No reference counterpart: measurement only.) */
+ROIFT_API int roift_kernel_timings(const roift_ctx *ctx, double *out, int n);
 /* Wall-clock breakdown of the last OIFT's persistent kernel, measured on the
    device between its grid barriers: out[0..8] milliseconds spent in the phases
    level-scan, collect, key-collect, key-rank, flood-seed, flood-round,

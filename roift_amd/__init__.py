@@ -24,7 +24,7 @@ NIL = -1
 
 EXPORTS = (
     "roift_create", "roift_destroy", "roift_last_error", "roift_set_stream", "roift_last_timings",
-    "roift_oift_profile", "roift_set_param",
+    "roift_kernel_timings", "roift_oift_profile", "roift_set_param",
     "roift_weights_dev", "roift_oift_fmax_dev", "roift_oift_edges_dev", "roift_relax_dev",
     "roift_pipeline_dev", "roift_oift_fmax", "roift_relax", "oift_fmax_gpu_run",
 )
@@ -61,6 +61,7 @@ def lib():
         L.roift_last_error.restype = C.c_char_p
         L.roift_set_stream.argtypes = [vp, vp]
         L.roift_last_timings.argtypes = [vp, dp]
+        L.roift_kernel_timings.argtypes = [vp, dp, C.c_int]
         L.roift_oift_profile.argtypes = [vp, dp, ip]
         L.roift_set_param.argtypes = [vp, ip, ip]
         L.roift_weights_dev.argtypes = [vp, pp, vp, ip, vp, vp, ip, i32p, i32p]
@@ -131,6 +132,13 @@ class Context:
         self._check(self._L.roift_last_timings(self._h, out))
         keys = ("weights_ms", "oift_ms", "relax_ms", "labels_ms", "oift_steps", "flood_rounds")
         return dict(zip(keys, list(out)))
+
+    def kernel_timings(self):
+        """Device-event ms of the last call's main kernels (weights, OIFT, relaxation loop)."""
+        out = (C.c_double * 4)()
+        self._check(self._L.roift_kernel_timings(self._h, out, 4))
+        return {"weights_ms": out[0], "oift_kernel_ms": out[1], "relax_loop_ms": out[2],
+                "relax_launches": int(out[3])}
 
     PARAMS = {"oift_blocks": 1, "flood_budget": 2, "relax_depth": 3}
 

@@ -131,16 +131,6 @@ Weights weights_into_ctx(roift_ctx &c, const Dims &d, int nadj, const void *img,
     return w;
 }
 
-void ev_record(roift_ctx &c, int i) { ROIFT_HIP_CHECK(hipEventRecord(c.ev[i], c.stream)); }
-
-double ev_ms(roift_ctx &c, int a, int b)
-{
-    float ms = 0.f;
-    ROIFT_HIP_CHECK(hipEventSynchronize(c.ev[b]));
-    ROIFT_HIP_CHECK(hipEventElapsedTime(&ms, c.ev[a], c.ev[b]));
-    return ms;
-}
-
 }  // namespace
 
 extern "C" {
@@ -194,6 +184,13 @@ ROIFT_API int roift_last_timings(const roift_ctx *c, double *out6)
 {
     if (!c || !out6) return ROIFT_EINVAL;
     for (int i = 0; i < 6; i++) out6[i] = c->timings[i];
+    return ROIFT_OK;
+}
+
+ROIFT_API int roift_kernel_timings(const roift_ctx *c, double *out, int n)
+{
+    if (!c || !out || n < 4) return ROIFT_EINVAL;
+    for (int i = 0; i < 4; i++) out[i] = c->ktimes[i];
     return ROIFT_OK;
 }
 

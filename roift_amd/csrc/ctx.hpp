@@ -18,7 +18,10 @@ struct roift_ctx {
     int tune_blocks = 0;            // OIFT persistent grid size (0: one workgroup per CU)
     int tune_budget = 0;            // OIFT LDS batches per workgroup per flood round (0: default)
     int tune_relax_depth = 0;       // relaxation temporal-block depth (0: default)
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[16] = {};
+    // device-event times of the last call's main kernels, ms: weight builder, OIFT
+    // persistent kernel, all relaxation iterations; [3] = relaxation launches
+    double ktimes[4] = {0, 0, 0, 0};
 };
 
 namespace roift {
@@ -40,6 +43,17 @@ T *buf(roift_ctx &c, const std::string &name, size_t count)
     if (hipMalloc(&p, bytes) != hipSuccess) throw Error{ROIFT_ENOMEM, "hipMalloc failed for " + name};
     c.bufs[name] = {p, bytes};
     return static_cast<T *>(p);
+}
+
+// Device events on the context's stream (slots 0-7: stages, 8-15: kernels).
+inline void ev_record(roift_ctx &c, int i) { ROIFT_HIP_CHECK(hipEventRecord(c.ev[i], c.stream)); }
+
+inline double ev_ms(roift_ctx &c, int a, int b)
+{
+    float ms = 0.f;
+    ROIFT_HIP_CHECK(hipEventSynchronize(c.ev[b]));
+    ROIFT_HIP_CHECK(hipEventElapsedTime(&ms, c.ev[a], c.ev[b]));
+    return ms;
 }
 
 inline unsigned grid_for(long long n, int block = 256, long long cap = 8192)

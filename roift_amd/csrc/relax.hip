@@ -448,6 +448,7 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
         ROIFT_HIP_CHECK(hipGetLastError());
     }
     const dim3 mgrid((d.xs + MX - 1) / MX, (d.ys + MY - 1) / MY, (d.zs + MZ - 1) / MZ);
+    ev_record(c, 12);
     for (int it = 0; it < niter; it++) {
         int32_t *lo = (it == niter - 1) ? d_label : nullptr;
         if (march) {
@@ -462,6 +463,7 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
         }
         std::swap(fin, fout);
     }
+    ev_record(c, 13);
     if (d_memb) {
         if (nc == 2)
             hipLaunchKernelGGL(k_memb_out2, dim3(G), dim3(256), 0, st, n, fin, bg, d_memb);
@@ -470,6 +472,8 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
         ROIFT_HIP_CHECK(hipGetLastError());
     }
     ROIFT_HIP_CHECK(hipStreamSynchronize(st));
+    c.ktimes[2] = ev_ms(c, 12, 13);
+    c.ktimes[3] = niter;
     return 0;
 }
 

@@ -125,6 +125,7 @@ WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_im
     // per_pq / 100.0 in double, as the reference's "w *= (1.0 +- per_pq/100.0)"
     const double fp = 1.0 + (double)per / 100.0, fm = 1.0 - (double)per / 100.0;
     const bool i16 = image_dtype == ROIFT_I16, w16 = weight_dtype == ROIFT_U16;
+    ev_record(c, 8);
     if (nadj == 7) {
         if (i16 && w16) launch_weights<7, int16_t, uint16_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
         else if (i16) launch_weights<7, int16_t, uint32_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
@@ -136,9 +137,11 @@ WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_im
         else if (w16) launch_weights<27, int32_t, uint16_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
         else launch_weights<27, int32_t, uint32_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
     }
+    ev_record(c, 9);
     int host[4];
     ROIFT_HIP_CHECK(hipMemcpyAsync(host, ext, sizeof(host), hipMemcpyDeviceToHost, c.stream));
     ROIFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+    c.ktimes[0] = ev_ms(c, 8, 9);
     ROIFT_REQUIRE(host[2] == 0, ROIFT_EUNSUPPORTED, "arc weight exceeds the weight dtype range (use U32 weights)");
     return WeightInfo{host[0], host[1]};
 }
