@@ -118,9 +118,9 @@ __global__ void k_init(State s, long long n, long long nkeys)
 {
     for (long long p = gtid(); p < n; p += gstride()) {
         s.word[p] = INF64;
-        s.run[p] = INF64;
-
-        s.flags[p] = (s.label[p] != NIL) ? FL_FROZEN : 0;
+        const bool frozen = s.label[p] != NIL;
+        s.run[p] = frozen ? 0ull : INF64;   // frozen: no flood enters (see the rank-word notes)
+        s.flags[p] = frozen ? FL_FROZEN : 0;
         s.tstamp[p] = 0;
         s.qstamp[p] = 0;
         s.estamp[p] = 0;
@@ -151,6 +151,7 @@ __global__ void k_seed_check(State s, const int32_t *__restrict__ S, int ns, lon
             else if (label_class(s, s.label[v]) < 0) atomicOr(&s.ctr[K_ERR], 8u);
             if (atomicAdd(&s.estamp[v], 1u) != 0) atomicOr(&s.ctr[K_ERR], 4u);
             s.flags[v] = 0;
+            s.run[v] = INF64;
             if (i > 0) {
                 const int u = S[i - 1];
                 const int lu = (u >= 0 && u < n) ? s.label[u] : NIL;
