@@ -120,6 +120,44 @@ ROIFT_API int roift_relax_dev(roift_ctx *ctx, const roift_problem *pb, const voi
                               const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter, float *d_memb,
                               int32_t *h_nclasses);
 
+/* Drop-in for gft::ift::ORelax_1, the binary fallback ORelax_1_Multi takes when S
+ * is empty (gft_ift.cpp:8678-8682), and for its per-voxel-weight forms
+ * ORelax_1(W, ...) and ORelax_1(Wx, Wy, Wz, ...) (gft_ift.h:443-482,
+ * gft_ift.cpp:8498-8653, 9189-9498).  One channel f = label/Imax, hard polarity,
+ * thresholded at 0.5: d_label in = labels, out = 0/1.  Weight base: the image
+ * (d_W = d_Wx = NULL), int32 d_W (W_p + W_q), or int32 d_Wx/d_Wy/d_Wz (per axis,
+ * Spheric(1.0) whatever pb->adj_radius says).  Seeds (n_seeds may be 0) keep their
+ * initial value.  d_memb (nullable) receives f (N floats).  roift_relax_dev with
+ * n_seeds == 0 runs the image form, as ORelax_1_Multi does.                      */
+ROIFT_API int roift_relax_binary_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                     const int32_t *d_W, const int32_t *d_Wx, const int32_t *d_Wy,
+                                     const int32_t *d_Wz, const int32_t *d_seeds, int n_seeds, int32_t *d_label,
+                                     int niter, float *d_memb);
+
+/* ---- experiment variants (experiments/exp_gradient_weight.cpp, exp_gaussian_rbf_relax.cpp) */
+/* OIFT_Multi_GradWeight (exp_gradient_weight.cpp:196-270): the OIFT with arc weight
+ * (int)(wf (1 +- per/100)), wf = alpha|I_p-I_q| + (1-alpha)(G_p+G_q)/2 in float,
+ * clamped at 0.  d_grad: int32 normalised gradient image (the experiment scales a
+ * Sobel magnitude to [0, max I]).  pb->per only (no per-class form).  Exact.     */
+ROIFT_API int roift_oift_gradweight_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                        const int32_t *d_grad, float alpha, const int32_t *d_seeds, int n_seeds,
+                                        int32_t *d_label, int32_t *d_cost);
+/* ORelax_1_Multi_GradWeight (exp_gradient_weight.cpp:272-504): ORelax_1_Multi
+ * with the blended base weight, a clamp at 0 after the polarity factor and a
+ * float Wmax.  Arguments as roift_relax_dev; leaves d_label unchanged when
+ * n_seeds == 0, as the experiment does.                                        */
+ROIFT_API int roift_relax_gradweight_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image,
+                                         int image_dtype, const int32_t *d_grad, float alpha, const int32_t *d_seeds,
+                                         int n_seeds, int32_t *d_label, int niter, float *d_memb);
+/* ORelax_1_Multi_GaussRBF (exp_gaussian_rbf_relax.cpp:211-427): w = exp(-d^2 /
+ * sigma^2) / Dpq, no polarity.  sigma <= 0 uses estimate_sigma
+ * (exp_gaussian_rbf_relax.cpp:194-208, the image's global standard deviation);
+ * h_sigma (nullable) receives the sigma used.  Leaves d_label unchanged when
+ * n_seeds == 0.                                                                  */
+ROIFT_API int roift_relax_rbf_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                  float sigma, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter,
+                                  float *d_memb, float *h_sigma);
+
 /* ---- the whole hot path ------------------------------------------------------------- */
 /* OIFT_Multi followed by ORelax_1_Multi on the same inputs, as the reference
  * CLI runs them (oiftrelax.cpp:379-401): weights + OIFT + relax + labels.     */
@@ -132,6 +170,9 @@ ROIFT_API int roift_oift_fmax(roift_ctx *ctx, const roift_problem *pb, const int
                               const int32_t *h_seeds, int n_seeds, int32_t *h_label, int32_t *h_cost);
 ROIFT_API int roift_relax(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image,
                           const int32_t *h_seeds, int n_seeds, int32_t *h_label, int niter, float *h_memb);
+ROIFT_API int roift_relax_binary(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image, const int32_t *h_W,
+                                 const int32_t *h_Wx, const int32_t *h_Wy, const int32_t *h_Wz,
+                                 const int32_t *h_seeds, int n_seeds, int32_t *h_label, int niter, float *h_memb);
 
 /* ---- legacy reference C-ABI ---------------------------------------------------------- */
 /* Same symbol and signature as the reference's GPU entry point

@@ -6,6 +6,7 @@
 //   gft::ift::OIFT_Multi_PerClass     (gft_ift.h:117-122)
 //   gft::ift::ORelax_1_Multi          (gft_ift.h:451-456)
 //   gft::ift::ORelax_1_Multi_PerClass (gft_ift.h:459-465)
+//   gft::ift::ORelax_1 and its W / Wx,Wy,Wz forms (gft_ift.h:443-482)
 // Each copies the scene to the GPU, runs libroift.so (include/roift.h) and writes
 // the label scene back in place, exactly as the originals modify `label`.
 //
@@ -54,6 +55,10 @@ void OIFT_Multi_PerClass(sAdjRel3 *A, sScene32 *scn, const float *per_class, int
 void ORelax_1_Multi(sAdjRel3 *A, sScene32 *scn, float per, int *S, sScene32 *label, int ntimes);
 void ORelax_1_Multi_PerClass(sAdjRel3 *A, sScene32 *scn, const float *per_class, int max_label, int *S,
                              sScene32 *label, int ntimes);
+void ORelax_1(sAdjRel3 *A, sScene32 *scn, float per, int *S, sScene32 *label, int ntimes);
+void ORelax_1(sScene32 *W, sAdjRel3 *A, sScene32 *scn, float per, int *S, sScene32 *label, int ntimes);
+void ORelax_1(sScene32 *Wx, sScene32 *Wy, sScene32 *Wz, sScene32 *scn, float per, int *S, sScene32 *label,
+              int ntimes);
 }  // namespace ift
 
 }  // namespace gft

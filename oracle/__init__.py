@@ -56,6 +56,16 @@ def lib():
                                     C.c_int, i32p, i32p, C.c_int, C.c_int, f64p, i32p, i32p]
         L.orc_relax_binary_f32.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, i32p, i32p,
                                            C.c_int, f32p]
+        L.orc_relax_binary_w_f32.argtypes = [i32p, i32p, i32p, i32p, i32p, C.c_int, C.c_int, C.c_int, C.c_float,
+                                             C.c_float, i32p, i32p, C.c_int, C.c_int, f32p]
+        for name, mp in (("orc_relax_model_f32", f32p), ("orc_relax_model_f64", f64p)):
+            getattr(L, name).argtypes = [i32p, i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float,
+                                         C.c_float, C.c_int, C.c_float, f32p, C.c_int, i32p, i32p, C.c_int,
+                                         C.c_int, mp, i32p, i32p]
+        L.orc_gradweight_edges.argtypes = [i32p, i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                           u32p, u32p]
+        L.orc_estimate_sigma.argtypes = [i32p, C.c_int64]
+        L.orc_estimate_sigma.restype = C.c_float
         _LIB = L
     return _LIB
 
@@ -75,6 +85,16 @@ def ref_lib():
                                 i32p, i32p, C.c_int]
         L.ref_pipeline.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, i32p, i32p, C.c_int,
                                    f64p, f64p]
+        L.ref_relax_binary.argtypes = [i32p, i32p, i32p, i32p, i32p, C.c_int, C.c_int, C.c_int, C.c_float,
+                                       C.c_float, i32p, i32p, C.c_int]
+        L.ref_oift_gradweight.argtypes = [i32p, i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                          i32p, i32p]
+        L.ref_relax_gradweight.argtypes = [i32p, i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                           i32p, i32p, C.c_int]
+        L.ref_relax_rbf.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, i32p, i32p,
+                                    C.c_int]
+        L.ref_estimate_sigma.argtypes = [i32p, C.c_int, C.c_int, C.c_int]
+        L.ref_estimate_sigma.restype = C.c_float
         _REF = L
     return _REF
 
@@ -197,6 +217,69 @@ def relax_binary(I, seeds, label, niter, adj_r=1.0, per=0.0):
     return dict(label=lab.reshape(I.shape), memb=memb)
 
 
+RELAX_MODELS = {"intensity": 0, "grad": 1, "rbf": 2}
+
+
+def relax_model(I, seeds, label, niter, model, G=None, alpha=0.5, sigma=1.0, adj_r=1.0, per=0.0, fp64=False,
+                dense=False):
+    """Multi-class relaxation with the GradWeight or Gaussian-RBF weight model restated
+    (see orc_relax_model_f32).  Returns dict(label, memb[nc, ...], classes)."""
+    I = _i32(I)
+    xs, ys, zs = _dims(I)
+    lab = _i32(label).copy()
+    S = _gft_S(seeds)
+    Gi = None if G is None else _i32(G)
+    nc = np.zeros(1, np.int32); cls = np.zeros(64, np.int32)
+    present = {0} | {int(lab.ravel()[s]) for s in np.asarray(seeds).ravel() if lab.ravel()[s] >= 0}
+    memb = np.empty((len(present),) + I.shape, np.float64 if fp64 else np.float32)
+    fn = lib().orc_relax_model_f64 if fp64 else lib().orc_relax_model_f32
+    rc = fn(_p(I, C.c_int32), _p(Gi, C.c_int32), xs, ys, zs, adj_r, RELAX_MODELS[model], alpha, sigma, 0, per,
+            None, -1, _p(S, C.c_int32), _p(lab, C.c_int32), int(niter), int(dense),
+            _p(memb, C.c_double if fp64 else C.c_float), _p(cls, C.c_int32), _p(nc, C.c_int32))
+    if rc != 0:
+        raise ValueError(f"orc_relax_model failed rc={rc}")
+    return dict(label=lab.reshape(I.shape), memb=memb[: nc[0]], classes=cls[: nc[0]].copy())
+
+
+def relax_binary_w(I, seeds, label, niter, adj_r=1.0, per=0.0, W=None, Wxyz=None, dense=False):
+    """ORelax_1 in its image, per-voxel-weight (W) or per-axis (Wxyz) form."""
+    I = _i32(I)
+    xs, ys, zs = _dims(I)
+    lab = _i32(label).copy()
+    S = None if seeds is None else _gft_S(seeds)
+    Wi = None if W is None else _i32(W)
+    Wx, Wy, Wz = (None, None, None) if Wxyz is None else tuple(_i32(w) for w in Wxyz)
+    memb = np.empty(I.shape, np.float32)
+    rc = lib().orc_relax_binary_w_f32(_p(I, C.c_int32), _p(Wi, C.c_int32), _p(Wx, C.c_int32), _p(Wy, C.c_int32),
+                                      _p(Wz, C.c_int32), xs, ys, zs, adj_r, per, _p(S, C.c_int32),
+                                      _p(lab, C.c_int32), int(niter), int(dense), _p(memb, C.c_float))
+    assert rc == 0
+    return dict(label=lab.reshape(I.shape), memb=memb)
+
+
+def gradweight_edges(I, G, adj_r=1.0, alpha=0.5, per=0.0):
+    """(wa, wb) of OIFT_Multi_GradWeight in the orc_edge_weights layout."""
+    I = _i32(I); Gi = _i32(G)
+    xs, ys, zs = _dims(I)
+    E = (spheric(adj_r).shape[0] - 1) // 2
+    wa = np.empty((E,) + I.shape, np.uint32); wb = np.empty((E,) + I.shape, np.uint32)
+    rc = lib().orc_gradweight_edges(_p(I, C.c_int32), _p(Gi, C.c_int32), xs, ys, zs, adj_r, alpha, per,
+                                    _p(wa, C.c_uint32), _p(wb, C.c_uint32))
+    assert rc == 0
+    return wa, wb
+
+
+def oift_gradweight(I, G, seeds, label, adj_r=1.0, alpha=0.5, per=0.0):
+    """OIFT_Multi_GradWeight restated: grad-model edge weights + the keyed-heap OIFT."""
+    wa, wb = gradweight_edges(I, G, adj_r, alpha, per)
+    return oift_edges(wa, wb, seeds, label, adj_r, pol_sign=np.array([-1], np.int8))
+
+
+def estimate_sigma(I):
+    I = _i32(I)
+    return float(lib().orc_estimate_sigma(_p(I, C.c_int32), I.size))
+
+
 # ---- the compiled reference (only where oracle/_ref was built) -------------------
 
 def ref_oift(I, seeds, label, adj_r=1.0, per=0.0, per_class=None):
@@ -231,3 +314,47 @@ def ref_pipeline(I, seeds, label, niter, adj_r=1.0, per=0.0):
     L.ref_pipeline(_p(I, C.c_int32), xs, ys, zs, adj_r, per, _p(S, C.c_int32), _p(lab, C.c_int32),
                    int(niter), _p(t1, C.c_double), _p(t2, C.c_double))
     return lab, float(t1[0]), float(t2[0])
+
+
+def ref_relax_binary(I, seeds, label, niter, adj_r=1.0, per=0.0, W=None, Wxyz=None):
+    L = ref_lib()
+    I = _i32(I); xs, ys, zs = _dims(I)
+    lab = _i32(label).copy()
+    S = _gft_S(np.zeros(0, np.int32) if seeds is None else seeds)
+    Wi = None if W is None else _i32(W)
+    Wx, Wy, Wz = (None, None, None) if Wxyz is None else tuple(_i32(w) for w in Wxyz)
+    L.ref_relax_binary(_p(I, C.c_int32), _p(Wi, C.c_int32), _p(Wx, C.c_int32), _p(Wy, C.c_int32),
+                       _p(Wz, C.c_int32), xs, ys, zs, adj_r, per, _p(S, C.c_int32), _p(lab, C.c_int32), int(niter))
+    return lab
+
+
+def ref_oift_gradweight(I, G, seeds, label, adj_r=1.0, alpha=0.5, per=0.0):
+    L = ref_lib()
+    I = _i32(I); Gi = _i32(G); xs, ys, zs = _dims(I)
+    lab = _i32(label).copy()
+    L.ref_oift_gradweight(_p(I, C.c_int32), _p(Gi, C.c_int32), xs, ys, zs, adj_r, per, alpha,
+                          _p(_gft_S(seeds), C.c_int32), _p(lab, C.c_int32))
+    return lab
+
+
+def ref_relax_gradweight(I, G, seeds, label, niter, adj_r=1.0, alpha=0.5, per=0.0):
+    L = ref_lib()
+    I = _i32(I); Gi = _i32(G); xs, ys, zs = _dims(I)
+    lab = _i32(label).copy()
+    L.ref_relax_gradweight(_p(I, C.c_int32), _p(Gi, C.c_int32), xs, ys, zs, adj_r, per, alpha,
+                           _p(_gft_S(seeds), C.c_int32), _p(lab, C.c_int32), int(niter))
+    return lab
+
+
+def ref_relax_rbf(I, seeds, label, niter, sigma, adj_r=1.0, per=0.0):
+    L = ref_lib()
+    I = _i32(I); xs, ys, zs = _dims(I)
+    lab = _i32(label).copy()
+    L.ref_relax_rbf(_p(I, C.c_int32), xs, ys, zs, adj_r, per, sigma, _p(_gft_S(seeds), C.c_int32),
+                    _p(lab, C.c_int32), int(niter))
+    return lab
+
+
+def ref_estimate_sigma(I):
+    I = _i32(I); xs, ys, zs = _dims(I)
+    return float(ref_lib().ref_estimate_sigma(_p(I, C.c_int32), xs, ys, zs))

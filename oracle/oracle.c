@@ -23,8 +23,13 @@
  *   orc_relax            -- golden fixtures (labels), oracle/_ref (labels),
  *                           closed forms on constant images, invariants,
  *                           masked == dense equality.
- *   orc_relax_binary     -- oracle/_ref labels when built; otherwise "parity
- *                           unpinned" beyond the constant-image closed form.
+ *   orc_relax_model      -- GRAD/RBF weight models: oracle/_ref labels (the
+ *                           experiments compiled from their own sources),
+ *                           golden fixtures, FP32 vs FP64, masked == dense.
+ *   orc_gradweight_edges -- orc_oift_edges on them == the compiled
+ *                           OIFT_Multi_GradWeight labels; golden fixtures.
+ *   orc_relax_binary_w   -- oracle/_ref labels (ORelax_1 image / W / Wxyz
+ *                           forms); golden fixtures; constant-image closed form.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -300,6 +305,62 @@ int orc_edge_weights(const int32_t *I, int xs, int ys, int zs, float adj_r, floa
 }
 
 /* ------------------------------------------------------------------------ */
+/* Oriented edge weights of OIFT_Multi_GradWeight (exp_gradient_weight.cpp:  */
+/* 244-254): base wf = alpha|I_p-I_q| + (1-alpha)(G_p+G_q)*0.5 in float,      */
+/* w = (int)(wf (1 +- per_pq/100)) in float, (int)wf when I_p == I_q, and a   */
+/* clamp at 0.  wf is symmetric in p and q and (1 - (-per)/100) == (1 +       */
+/* per/100) exactly, so w_bg(p->q) = w_obj(q->p) holds as for OIFT_Multi and  */
+/* the same (wa, wb) layout serves orc_oift_edges.                            */
+/* ------------------------------------------------------------------------ */
+static inline int grad_arc_weight(int Ip, int Iq, int Gp, int Gq, float alpha, float per_pq)
+{
+    float idf = (float)abs(Ip - Iq);
+    float gav = (float)(Gp + Gq) * 0.5f;
+    float wf = alpha * idf + (1.0f - alpha) * gav;
+    int w = (int)wf;
+    if (Ip > Iq) w = (int)(wf * (1.0f + per_pq / 100.0f));
+    else if (Ip < Iq) w = (int)(wf * (1.0f - per_pq / 100.0f));
+    if (w < 0) w = 0;
+    return w;
+}
+
+int orc_gradweight_edges(const int32_t *I, const int32_t *G, int xs, int ys, int zs, float adj_r, float alpha,
+                         float per, uint32_t *wa, uint32_t *wb)
+{
+    orc_adj A;
+    if (adj_make(adj_r, &A) != ORC_OK) return ORC_EINVAL;
+    const int half = (A.n - 1) / 2;
+    const int64_t n = (int64_t)xs * ys * zs;
+    const int xy = xs * ys;
+    for (int s = 0; s < half; s++) {
+        int i = half + 1 + s;
+        for (int64_t p = 0; p < n; p++) {
+            int px = (int)((p % xy) % xs), py = (int)((p % xy) / xs), pz = (int)(p / xy);
+            int qx = px + A.dx[i], qy = py + A.dy[i], qz = pz + A.dz[i];
+            if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) {
+                wa[s * n + p] = 0xFFFFFFFFu; wb[s * n + p] = 0xFFFFFFFFu; continue;
+            }
+            int64_t q = qx + (int64_t)qy * xs + (int64_t)qz * xy;
+            wa[s * n + p] = (uint32_t)grad_arc_weight(I[p], I[q], G[p], G[q], alpha, per);
+            wb[s * n + p] = (uint32_t)grad_arc_weight(I[q], I[p], G[q], G[p], alpha, per);
+        }
+    }
+    return ORC_OK;
+}
+
+/* Global standard deviation of the image, as estimate_sigma                  */
+/* (exp_gaussian_rbf_relax.cpp:194-208): integer sums, double moments, var>=1. */
+float orc_estimate_sigma(const int32_t *I, int64_t n)
+{
+    long long sum = 0, sum2 = 0;
+    for (int64_t p = 0; p < n; p++) { long long v = I[p]; sum += v; sum2 += v * v; }
+    double mean = (double)sum / (double)n;
+    double var = (double)sum2 / (double)n - mean * mean;
+    if (var < 1.0) var = 1.0;
+    return (float)sqrt(var);
+}
+
+/* ------------------------------------------------------------------------ */
 /* The same Prim-style OIFT driven by explicit oriented edge weights (wa,wb  */
 /* as produced above, or tie-free ranks), with a lazy binary heap keyed by   */
 /* (value, insertion sequence).  Popping the minimum (value, seq) is the      */
@@ -429,13 +490,13 @@ done:
 /* dense:   1 updates every voxel every iteration instead of the dilating    */
 /*          band (bit-identical, used to test that claim).                    */
 /* ------------------------------------------------------------------------ */
-#define RELAX_BODY(REAL, FABS, SQRT)                                                          \
+#define RELAX_BODY(REAL, FABS, SQRT, EXP)                                                        \
     orc_adj A;                                                                                \
     if (adj_make(adj_r, &A) != ORC_OK) return ORC_EINVAL;                                     \
     const int64_t n64 = (int64_t)xs * ys * zs;                                                \
     if (n64 > INT_MAX || n64 <= 0) return ORC_EINVAL;                                         \
     const int n = (int)n64;                                                                   \
-    if (!S || S[0] <= 0) return ORC_EINVAL;                                                   \
+    if (!S || S[0] <= 0) return (model == ORC_RELAX_INTENSITY) ? ORC_EINVAL : ORC_OK;       \
     int max_seed_label = 0;                                                                   \
     for (int i = 1; i <= S[0]; i++) {                                                         \
         int p = S[i];                                                                         \
@@ -473,7 +534,12 @@ done:
     }                                                                                         \
     int Imax = INT_MIN;                                                                       \
     for (int p = 0; p < n; p++) if (I[p] > Imax) Imax = I[p];                                  \
-    REAL Wmax = (REAL)((double)(float)Imax * (1.0 + (double)max_abs / 100.0));                \
+    /* ORelax_1_Multi: float Wmax = Imax; Wmax *= (1.0 + |per|/100.0) (double product);      \
+       GradWeight: the same in float arithmetic (exp_gradient_weight.cpp:341-342).          */ \
+    REAL Wmax = (model == ORC_RELAX_GRAD)                                                     \
+        ? (REAL)((REAL)(float)Imax * ((REAL)1 + (REAL)FABS(per) / (REAL)100))                 \
+        : (REAL)((double)(float)Imax * (1.0 + (double)max_abs / 100.0));                      \
+    const REAL sigma2 = (REAL)sigma * (REAL)sigma;                                            \
     const size_t total = (size_t)n * (size_t)nc;                                              \
     REAL *f1 = (REAL *)calloc(total, sizeof(REAL));                                           \
     REAL *f2 = (REAL *)calloc(total, sizeof(REAL));                                           \
@@ -509,16 +575,29 @@ done:
                 if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) continue; \
                 int q = qx + qy * xs + qz * xy;                                               \
                 size_t qo = (size_t)q * nc;                                                   \
-                REAL w = FABS((REAL)(I[p] - I[q]));                                           \
-                REAL s;                                                                       \
-                if (use_per_class) { s = (REAL)0; for (int k = 0; k < nc; k++) s += pcm[k] * f1[qo + k]; } \
-                else s = (REAL)per * ((REAL)2 * f1[qo + bg] - (REAL)1);                       \
-                if (I[p] > I[q]) w *= ((REAL)1 + s / (REAL)100);                               \
-                else if (I[p] < I[q]) w *= ((REAL)1 - s / (REAL)100);                          \
-                w = Wmax - w;                                                                 \
-                if (w < (REAL)0) w = (REAL)0;                                                 \
-                w = w * w; w = w * w; w = w * w;                                              \
-                w /= Dpq[i];                                                                  \
+                REAL w;                                                                       \
+                if (model == ORC_RELAX_RBF) {                                                 \
+                    REAL diff = (REAL)(I[p] - I[q]);                                          \
+                    w = EXP(-(diff * diff) / sigma2) / Dpq[i];                                \
+                } else {                                                                      \
+                    if (model == ORC_RELAX_GRAD) {                                            \
+                        REAL idf = FABS((REAL)(I[p] - I[q]));                                 \
+                        REAL gav = (REAL)(G[p] + G[q]) * (REAL)0.5;                           \
+                        w = (REAL)alpha * idf + ((REAL)1 - (REAL)alpha) * gav;                \
+                    } else {                                                                  \
+                        w = FABS((REAL)(I[p] - I[q]));                                        \
+                    }                                                                         \
+                    REAL s;                                                                   \
+                    if (use_per_class) { s = (REAL)0; for (int k = 0; k < nc; k++) s += pcm[k] * f1[qo + k]; } \
+                    else s = (REAL)per * ((REAL)2 * f1[qo + bg] - (REAL)1);                   \
+                    if (I[p] > I[q]) w *= ((REAL)1 + s / (REAL)100);                           \
+                    else if (I[p] < I[q]) w *= ((REAL)1 - s / (REAL)100);                      \
+                    if (model == ORC_RELAX_GRAD && w < (REAL)0) w = (REAL)0;                  \
+                    w = Wmax - w;                                                             \
+                    if (w < (REAL)0) w = (REAL)0;                                             \
+                    w = w * w; w = w * w; w = w * w;                                          \
+                    w /= Dpq[i];                                                              \
+                }                                                                             \
                 sw += w;                                                                      \
                 for (int k = 0; k < nc; k++) f2[po + k] += w * f1[qo + k];                    \
             }                                                                                 \
@@ -566,12 +645,42 @@ done:
     free(f1); free(f2); free(mnodes); free(mask); free(present); free(l2c);                   \
     return ORC_OK;
 
+/* Weight models of the multi-class relaxation:                                */
+/*  ORC_RELAX_INTENSITY  ORelax_1_Multi[_PerClass] (gft_ift.cpp:8800-8816)      */
+/*  ORC_RELAX_GRAD       ORelax_1_Multi_GradWeight: base alpha|dI| +            */
+/*                       (1-alpha)(G_p+G_q)/2, polarity, clamp at 0, float Wmax */
+/*                       (exp_gradient_weight.cpp:407-422)                      */
+/*  ORC_RELAX_RBF        ORelax_1_Multi_GaussRBF: w = exp(-d^2/sigma^2)/Dpq,    */
+/*                       no polarity (exp_gaussian_rbf_relax.cpp:343-345)       */
+/* GRAD and RBF leave the labels unchanged when S is empty, as the experiments  */
+/* do (exp_gradient_weight.cpp:296, exp_gaussian_rbf_relax.cpp:235).            */
+#define ORC_RELAX_INTENSITY 0
+#define ORC_RELAX_GRAD 1
+#define ORC_RELAX_RBF 2
+
+int orc_relax_model_f32(const int32_t *I, const int32_t *G, int xs, int ys, int zs, float adj_r, int model,
+                        float alpha, float sigma, int use_per_class, float per, const float *per_class,
+                        int max_label, const int32_t *S, int32_t *label, int ntimes, int dense,
+                        float *memb, int32_t *classes, int32_t *nclasses_out)
+{
+    RELAX_BODY(float, fabsf, sqrtf, expf)
+}
+
+int orc_relax_model_f64(const int32_t *I, const int32_t *G, int xs, int ys, int zs, float adj_r, int model,
+                        float alpha, float sigma, int use_per_class, float per, const float *per_class,
+                        int max_label, const int32_t *S, int32_t *label, int ntimes, int dense,
+                        double *memb, int32_t *classes, int32_t *nclasses_out)
+{
+    RELAX_BODY(double, fabs, sqrt, exp)
+}
+
 int orc_relax_f32(const int32_t *I, int xs, int ys, int zs, float adj_r,
                   int use_per_class, float per, const float *per_class, int max_label,
                   const int32_t *S, int32_t *label, int ntimes, int dense,
                   float *memb, int32_t *classes, int32_t *nclasses_out)
 {
-    RELAX_BODY(float, fabsf, sqrtf)
+    return orc_relax_model_f32(I, NULL, xs, ys, zs, adj_r, ORC_RELAX_INTENSITY, 0.0f, 1.0f, use_per_class, per,
+                               per_class, max_label, S, label, ntimes, dense, memb, classes, nclasses_out);
 }
 
 int orc_relax_f64(const int32_t *I, int xs, int ys, int zs, float adj_r,
@@ -579,7 +688,8 @@ int orc_relax_f64(const int32_t *I, int xs, int ys, int zs, float adj_r,
                   const int32_t *S, int32_t *label, int ntimes, int dense,
                   double *memb, int32_t *classes, int32_t *nclasses_out)
 {
-    RELAX_BODY(double, fabs, sqrt)
+    return orc_relax_model_f64(I, NULL, xs, ys, zs, adj_r, ORC_RELAX_INTENSITY, 0.0f, 1.0f, use_per_class, per,
+                               per_class, max_label, S, label, ntimes, dense, memb, classes, nclasses_out);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -587,20 +697,47 @@ int orc_relax_f64(const int32_t *I, int xs, int ys, int zs, float adj_r,
 /* f = label/Imax, hard polarity from f(q) < 0.5, no clamp, int Wmax.        */
 /* ORelax_1_Multi routes here when S is NULL or empty (gft_ift.cpp:8678).    */
 /* ------------------------------------------------------------------------ */
-int orc_relax_binary_f32(const int32_t *I, int xs, int ys, int zs, float adj_r, float per,
-                         const int32_t *S, int32_t *label, int ntimes, float *memb)
+/* Weight sources: W == NULL && Wx == NULL: the image form above (|I_p-I_q|,   */
+/* int Wmax = max I (1+|per|/100), divided by Dpq).  W != NULL: ORelax_1(W,...) */
+/* (gft_ift.cpp:9189-9331): base W[p]+W[q], int Wmax = 2 max W (1+|per|/100),  */
+/* no Dpq.  Wx != NULL: ORelax_1(Wx,Wy,Wz,...) (gft_ift.cpp:9333-9498) on       */
+/* Spheric(1.0): base W_axis[p]+W_axis[q] with the axis of the neighbour,       */
+/* Wmax = 2 max(max Wx, max Wy, max Wz) (1+|per|/100), no Dpq.                  */
+/* The polarity direction always follows the image.  dense: every voxel every   */
+/* iteration (equal to the band for dyadic label/Imax values).                  */
+int orc_relax_binary_w_f32(const int32_t *I, const int32_t *W, const int32_t *Wx, const int32_t *Wy,
+                           const int32_t *Wz, int xs, int ys, int zs, float adj_r, float per,
+                           const int32_t *S, int32_t *label, int ntimes, int dense, float *memb)
 {
     orc_adj A;
-    if (adj_make(adj_r, &A) != ORC_OK) return ORC_EINVAL;
+    if (adj_make(Wx ? 1.0f : adj_r, &A) != ORC_OK) return ORC_EINVAL;
     const int64_t n64 = (int64_t)xs * ys * zs;
     if (n64 > INT_MAX || n64 <= 0) return ORC_EINVAL;
+    if (Wx && (!Wy || !Wz)) return ORC_EINVAL;
     const int n = (int)n64, xy = xs * ys;
     float Dpq[125];
     for (int i = 1; i < A.n; i++)
         Dpq[i] = sqrtf((float)(A.dx[i] * A.dx[i]) + (float)(A.dy[i] * A.dy[i]) + (float)(A.dz[i] * A.dz[i]));
-    int Imax = INT_MIN, Imx = INT_MIN;
-    for (int p = 0; p < n; p++) { if (label[p] > Imax) Imax = label[p]; if (I[p] > Imx) Imx = I[p]; }
-    int Wmax = (int)((double)Imx * (1.0 + (double)fabsf(per) / 100.0));
+    int axis[125];   /* 0 = x, 1 = y, 2 = z (the last non-zero component wins, as gft_ift.cpp:9356-9364) */
+    for (int i = 1; i < A.n; i++) {
+        axis[i] = 0;
+        if (A.dx[i] != 0) axis[i] = 0;
+        if (A.dy[i] != 0) axis[i] = 1;
+        if (A.dz[i] != 0) axis[i] = 2;
+    }
+    const int32_t *Wa[3] = {Wx, Wy, Wz};
+    int Imax = INT_MIN, Bmax = INT_MIN;
+    for (int p = 0; p < n; p++) {
+        if (label[p] > Imax) Imax = label[p];
+        if (Wx) {
+            for (int t = 0; t < 3; t++) if (Wa[t][p] > Bmax) Bmax = Wa[t][p];
+        } else {
+            const int b = W ? W[p] : I[p];
+            if (b > Bmax) Bmax = b;
+        }
+    }
+    if (W || Wx) Bmax *= 2;
+    const int Wmax = (int)((double)Bmax * (1.0 + (double)fabsf(per) / 100.0));
     float *f1 = (float *)malloc(sizeof(float) * (size_t)n), *f2 = (float *)malloc(sizeof(float) * (size_t)n);
     int *mnodes = (int *)malloc(sizeof(int) * ((size_t)n + 1));
     char *mask = (char *)calloc((size_t)n, 1);
@@ -616,6 +753,7 @@ int orc_relax_binary_f32(const int32_t *I, int xs, int ys, int zs, float adj_r, 
             if (label[p] != label[q]) { mask[p] = 1; mnodes[++mnodes[0]] = p; break; }
         }
     }
+    if (dense) { mnodes[0] = n; for (int p = 0; p < n; p++) { mnodes[p + 1] = p; mask[p] = 1; } }
     int ninic = 1;
     while (ntimes > 0) {
         memcpy(f2, f1, sizeof(float) * (size_t)n);
@@ -628,13 +766,16 @@ int orc_relax_binary_f32(const int32_t *I, int xs, int ys, int zs, float adj_r, 
                 int qx = px + A.dx[i], qy = py + A.dy[i], qz = pz + A.dz[i];
                 if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) continue;
                 int q = qx + qy * xs + qz * xy;
-                float w = (float)abs(I[p] - I[q]);
+                float w;
+                if (Wx) w = (float)(Wa[axis[i]][p] + Wa[axis[i]][q]);
+                else if (W) w = (float)(W[p] + W[q]);
+                else w = (float)abs(I[p] - I[q]);
                 float per_pq = (f1[q] < 0.5f) ? per : -per;
                 if (I[p] > I[q]) w = (float)((double)w * (1.0 + (double)per_pq / 100.0));
                 else if (I[p] < I[q]) w = (float)((double)w * (1.0 - (double)per_pq / 100.0));
                 w = (float)Wmax - w;
                 w = w * w; w = w * w; w = w * w;
-                w /= Dpq[i];
+                if (!W && !Wx) w /= Dpq[i];
                 sw += w;
                 f2[p] += w * f1[q];
             }
@@ -643,7 +784,7 @@ int orc_relax_binary_f32(const int32_t *I, int xs, int ys, int zs, float adj_r, 
         float *t = f1; f1 = f2; f2 = t;
         if (S) for (int i = 1; i <= S[0]; i++) { int p = S[i]; f1[p] = (float)label[p] / (float)Imax; }
         ntimes--;
-        if (ntimes > 0) {
+        if (ntimes > 0 && !dense) {
             int nlast = mnodes[0];
             for (int j = ninic; j <= mnodes[0]; j++) {
                 int p = mnodes[j];
@@ -663,4 +804,10 @@ int orc_relax_binary_f32(const int32_t *I, int xs, int ys, int zs, float adj_r, 
     if (memb) memcpy(memb, f1, sizeof(float) * (size_t)n);
     free(f1); free(f2); free(mnodes); free(mask);
     return ORC_OK;
+}
+
+int orc_relax_binary_f32(const int32_t *I, int xs, int ys, int zs, float adj_r, float per,
+                         const int32_t *S, int32_t *label, int ntimes, float *memb)
+{
+    return orc_relax_binary_w_f32(I, NULL, NULL, NULL, NULL, xs, ys, zs, adj_r, per, S, label, ntimes, 0, memb);
 }

@@ -79,6 +79,37 @@ REF_API int ref_relax(const int *I, int xs, int ys, int zs, float adj_r,
     return 0;
 }
 
+// ORelax_1 binary forms (gft_ift.cpp:8498, 9189, 9333): W == Wx == NULL -> image form
+// on Spheric(adj_r); W -> per-voxel weight form; Wx/Wy/Wz -> per-axis form.
+// S may be NULL only for the forms that never read it; pass S[0] = 0 for none.
+REF_API int ref_relax_binary(const int *I, const int *W, const int *Wx, const int *Wy, const int *Wz, int xs, int ys,
+                             int zs, float adj_r, float per, const int *S, int *label, int ntimes)
+{
+    gft::sAdjRel3 *A = gft::AdjRel3::Spheric(adj_r);
+    gft::sScene32 *scn = make_scene(I, xs, ys, zs);
+    gft::sScene32 *lab = make_scene(label, xs, ys, zs);
+    int *Sc = make_S(S);
+    if (Wx) {
+        gft::sScene32 *sx = make_scene(Wx, xs, ys, zs), *sy = make_scene(Wy, xs, ys, zs), *sz = make_scene(Wz, xs, ys, zs);
+        gft::ift::ORelax_1(sx, sy, sz, scn, per, Sc, lab, ntimes);
+        gft::Scene32::Destroy(&sx);
+        gft::Scene32::Destroy(&sy);
+        gft::Scene32::Destroy(&sz);
+    } else if (W) {
+        gft::sScene32 *sw = make_scene(W, xs, ys, zs);
+        gft::ift::ORelax_1(sw, A, scn, per, Sc, lab, ntimes);
+        gft::Scene32::Destroy(&sw);
+    } else {
+        gft::ift::ORelax_1(A, scn, per, Sc, lab, ntimes);
+    }
+    std::memcpy(label, lab->data, sizeof(int) * (size_t)lab->n);
+    std::free(Sc);
+    gft::Scene32::Destroy(&lab);
+    gft::Scene32::Destroy(&scn);
+    gft::AdjRel3::Destroy(&A);
+    return 0;
+}
+
 // OIFT_Multi followed by ORelax_1_Multi on the same scenes, as oiftrelax.cpp:379-401
 // runs them; t_oift / t_relax receive steady_clock seconds around each call.
 REF_API int ref_pipeline(const int *I, int xs, int ys, int zs, float adj_r, float per,

@@ -1,0 +1,114 @@
+// ref_exp.cpp -- extern "C" wrappers (written for this project) around the
+// reference's experiment variants of the hot path, compiled from the experiment
+// sources where they lie (oracle/Makefile passes their paths as EXP_GRAD and
+// EXP_RBF).  Each experiment is a self-contained program whose functions are
+// file-static, so each is included into a namespace of its own here.
+//
+// TEST INFRASTRUCTURE ONLY (linked into oracle/_ref/libroift_ref.so): lets the
+// tests compare the oracle's restatements of
+//   OIFT_Multi_GradWeight / ORelax_1_Multi_GradWeight (exp_gradient_weight.cpp:196-504)
+//   ORelax_1_Multi_GaussRBF, estimate_sigma          (exp_gaussian_rbf_relax.cpp:194-427)
+// with the reference's own code.
+#include "gft.h"
+#include "gft_gradient3.h"
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace exp_grad {
+#include EXP_GRAD
+}
+namespace exp_rbf {
+#include EXP_RBF
+}
+
+namespace {
+
+gft::sScene32 *scene_of(const int *data, int xs, int ys, int zs)
+{
+    gft::sScene32 *s = gft::Scene32::Create(xs, ys, zs);
+    std::memcpy(s->data, data, sizeof(int) * (size_t)s->n);
+    return s;
+}
+
+int *S_of(const int *S)
+{
+    int *c = (int *)std::calloc((size_t)S[0] + 1, sizeof(int));
+    std::memcpy(c, S, sizeof(int) * ((size_t)S[0] + 1));
+    return c;
+}
+
+}  // namespace
+
+#define REF_API __attribute__((visibility("default")))
+
+extern "C" {
+
+// OIFT_Multi_GradWeight; label in/out.
+REF_API int ref_oift_gradweight(const int *I, const int *G, int xs, int ys, int zs, float adj_r, float per,
+                                float alpha, const int *S, int *label)
+{
+    gft::sAdjRel3 *A = gft::AdjRel3::Spheric(adj_r);
+    gft::sScene32 *scn = scene_of(I, xs, ys, zs), *g = scene_of(G, xs, ys, zs), *lab = scene_of(label, xs, ys, zs);
+    int *Sc = S_of(S);
+    exp_grad::OIFT_Multi_GradWeight(A, scn, g, per, alpha, Sc, lab);
+    std::memcpy(label, lab->data, sizeof(int) * (size_t)lab->n);
+    std::free(Sc);
+    gft::Scene32::Destroy(&lab);
+    gft::Scene32::Destroy(&g);
+    gft::Scene32::Destroy(&scn);
+    gft::AdjRel3::Destroy(&A);
+    return 0;
+}
+
+// ORelax_1_Multi_GradWeight; label in = OIFT labels, out = relaxed labels.
+REF_API int ref_relax_gradweight(const int *I, const int *G, int xs, int ys, int zs, float adj_r, float per,
+                                 float alpha, const int *S, int *label, int ntimes)
+{
+    gft::sAdjRel3 *A = gft::AdjRel3::Spheric(adj_r);
+    gft::sScene32 *scn = scene_of(I, xs, ys, zs), *g = scene_of(G, xs, ys, zs), *lab = scene_of(label, xs, ys, zs);
+    int *Sc = S_of(S);
+    exp_grad::ORelax_1_Multi_GradWeight(A, scn, g, per, alpha, Sc, lab, ntimes);
+    std::memcpy(label, lab->data, sizeof(int) * (size_t)lab->n);
+    std::free(Sc);
+    gft::Scene32::Destroy(&lab);
+    gft::Scene32::Destroy(&g);
+    gft::Scene32::Destroy(&scn);
+    gft::AdjRel3::Destroy(&A);
+    return 0;
+}
+
+// ORelax_1_Multi_GaussRBF; label in/out.
+REF_API int ref_relax_rbf(const int *I, int xs, int ys, int zs, float adj_r, float per, float sigma, const int *S,
+                          int *label, int ntimes)
+{
+    gft::sAdjRel3 *A = gft::AdjRel3::Spheric(adj_r);
+    gft::sScene32 *scn = scene_of(I, xs, ys, zs), *lab = scene_of(label, xs, ys, zs);
+    int *Sc = S_of(S);
+    exp_rbf::ORelax_1_Multi_GaussRBF(A, scn, per, sigma, Sc, lab, ntimes);
+    std::memcpy(label, lab->data, sizeof(int) * (size_t)lab->n);
+    std::free(Sc);
+    gft::Scene32::Destroy(&lab);
+    gft::Scene32::Destroy(&scn);
+    gft::AdjRel3::Destroy(&A);
+    return 0;
+}
+
+REF_API float ref_estimate_sigma(const int *I, int xs, int ys, int zs)
+{
+    gft::sScene32 *scn = scene_of(I, xs, ys, zs);
+    const float s = exp_rbf::estimate_sigma(scn);
+    gft::Scene32::Destroy(&scn);
+    return s;
+}
+
+}  // extern "C"

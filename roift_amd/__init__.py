@@ -27,6 +27,8 @@ EXPORTS = (
     "roift_kernel_timings", "roift_oift_profile", "roift_set_param",
     "roift_weights_dev", "roift_oift_fmax_dev", "roift_oift_edges_dev", "roift_relax_dev",
     "roift_pipeline_dev", "roift_oift_fmax", "roift_relax", "oift_fmax_gpu_run",
+    "roift_relax_binary_dev", "roift_oift_gradweight_dev", "roift_relax_gradweight_dev", "roift_relax_rbf_dev",
+    "roift_relax_binary",
 )
 
 
@@ -72,6 +74,12 @@ def lib():
         L.roift_oift_fmax.argtypes = [vp, pp, vp, vp, ip, vp, vp]
         L.roift_relax.argtypes = [vp, pp, vp, vp, ip, vp, ip, vp]
         L.oift_fmax_gpu_run.argtypes = [vp, vp, ip, ip, ip, ip, vp, ip, vp, ip]
+        fl, fp = C.c_float, C.POINTER(C.c_float)
+        L.roift_relax_binary_dev.argtypes = [vp, pp, vp, ip, vp, vp, vp, vp, vp, ip, vp, ip, vp]
+        L.roift_oift_gradweight_dev.argtypes = [vp, pp, vp, ip, vp, fl, vp, ip, vp, vp]
+        L.roift_relax_gradweight_dev.argtypes = [vp, pp, vp, ip, vp, fl, vp, ip, vp, ip, vp]
+        L.roift_relax_rbf_dev.argtypes = [vp, pp, vp, ip, fl, vp, ip, vp, ip, vp, fp]
+        L.roift_relax_binary.argtypes = [vp, pp, vp, vp, vp, vp, vp, vp, ip, vp, ip, vp]
         _lib = L
     return _lib
 
@@ -213,6 +221,67 @@ class Context:
                                             memb.data_ptr() if memb is not None else None, C.byref(nc)))
         return memb[: nc.value] if memb is not None else None
 
+    def relax_binary(self, image, seeds, label, niter, adj_r=1.0, per=0.0, W=None, Wxyz=None, want_memb=True):
+        """ORelax_1 (image, W or Wx/Wy/Wz form), in place on `label` (0/1 out); returns f or None."""
+        import torch
+        pb, _ = make_problem(image.shape, adj_r, per)
+        seeds = (torch.zeros(0, dtype=torch.int32, device=label.device) if seeds is None
+                 else seeds.to(device=label.device, dtype=torch.int32).contiguous())
+        memb = torch.empty(tuple(label.shape), dtype=torch.float32, device=label.device) if want_memb else None
+        ws = [None, None, None, None]
+        if W is not None:
+            ws[0] = W.to(torch.int32).contiguous()
+        if Wxyz is not None:
+            ws[1:] = [w.to(torch.int32).contiguous() for w in Wxyz]
+        ptr = [None if w is None else w.data_ptr() for w in ws]
+        self._check(self._L.roift_relax_binary_dev(self._h, C.byref(pb), image.data_ptr(), _dtype_code(image),
+                                                   ptr[0], ptr[1], ptr[2], ptr[3], seeds.data_ptr(),
+                                                   int(seeds.numel()), label.data_ptr(), int(niter),
+                                                   memb.data_ptr() if memb is not None else None))
+        return memb
+
+    def oift_gradweight(self, image, grad, seeds, label, alpha=0.5, adj_r=1.0, per=0.0, want_cost=True):
+        """OIFT_Multi_GradWeight in place on `label`; grad: int32 device tensor; returns the cost map."""
+        import torch
+        pb, _ = make_problem(image.shape, adj_r, per)
+        seeds = seeds.to(device=label.device, dtype=torch.int32).contiguous()
+        g = grad.to(torch.int32).contiguous()
+        cost = torch.empty_like(label) if want_cost else None
+        self._check(self._L.roift_oift_gradweight_dev(self._h, C.byref(pb), image.data_ptr(), _dtype_code(image),
+                                                      g.data_ptr(), float(alpha), seeds.data_ptr(),
+                                                      int(seeds.numel()), label.data_ptr(),
+                                                      cost.data_ptr() if cost is not None else None))
+        return cost
+
+    def relax_gradweight(self, image, grad, seeds, label, niter, alpha=0.5, adj_r=1.0, per=0.0, want_memb=True):
+        """ORelax_1_Multi_GradWeight in place on `label`; returns memberships [nclasses, ...] or None."""
+        import torch
+        pb, _ = make_problem(image.shape, adj_r, per)
+        seeds = seeds.to(device=label.device, dtype=torch.int32).contiguous()
+        g = grad.to(torch.int32).contiguous()
+        nc = _nclasses(label, seeds)
+        memb = torch.empty((nc,) + tuple(label.shape), dtype=torch.float32, device=label.device) if want_memb else None
+        self._check(self._L.roift_relax_gradweight_dev(self._h, C.byref(pb), image.data_ptr(), _dtype_code(image),
+                                                       g.data_ptr(), float(alpha), seeds.data_ptr(),
+                                                       int(seeds.numel()), label.data_ptr(), int(niter),
+                                                       memb.data_ptr() if memb is not None else None))
+        return memb
+
+    def relax_rbf(self, image, seeds, label, niter, sigma=0.0, adj_r=1.0, want_memb=True):
+        """ORelax_1_Multi_GaussRBF in place on `label`; sigma <= 0 uses the image's global std.
+        Returns (memberships or None, sigma used)."""
+        import torch
+        pb, _ = make_problem(image.shape, adj_r, 0.0)
+        seeds = seeds.to(device=label.device, dtype=torch.int32).contiguous()
+        nc = _nclasses(label, seeds)
+        memb = torch.empty((nc,) + tuple(label.shape), dtype=torch.float32, device=label.device) if want_memb else None
+        used = C.c_float(0.0)
+        self._check(self._L.roift_relax_rbf_dev(self._h, C.byref(pb), image.data_ptr(), _dtype_code(image),
+                                                float(sigma), seeds.data_ptr(), int(seeds.numel()),
+                                                label.data_ptr(), int(niter),
+                                                memb.data_ptr() if memb is not None else None, C.byref(used)))
+        return memb, used.value
+
     def pipeline(self, image, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None, cost=None, memb=None):
         """weights + OIFT + relax + labels on device tensors (label in/out)."""
         pb, _keep = make_problem(image.shape, adj_r, per, per_class)
@@ -232,6 +301,19 @@ class Context:
                                             lab.ctypes.data, None if cost is None else cost.ctypes.data))
         return lab, cost
 
+    def relax_binary_host(self, image, seeds, label, niter, adj_r=1.0, per=0.0, W=None, Wxyz=None):
+        img = np.ascontiguousarray(image, np.int32)
+        lab = np.ascontiguousarray(label, np.int32).copy()
+        S = np.ascontiguousarray(np.zeros(0) if seeds is None else seeds, np.int32)
+        ws = [None if W is None else np.ascontiguousarray(W, np.int32)]
+        ws += [None] * 3 if Wxyz is None else [np.ascontiguousarray(w, np.int32) for w in Wxyz]
+        memb = np.empty(img.shape, np.float32)
+        pb, _ = make_problem(img.shape, adj_r, per)
+        self._check(self._L.roift_relax_binary(self._h, C.byref(pb), img.ctypes.data,
+                                               *[None if w is None else w.ctypes.data for w in ws],
+                                               S.ctypes.data, S.size, lab.ctypes.data, int(niter), memb.ctypes.data))
+        return lab, memb
+
     def relax_host(self, image, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None):
         img = np.ascontiguousarray(image, np.int32)
         lab = np.ascontiguousarray(label, np.int32).copy()
@@ -241,6 +323,16 @@ class Context:
         self._check(self._L.roift_relax(self._h, C.byref(pb), img.ctypes.data, S.ctypes.data, S.size,
                                         lab.ctypes.data, int(niter), memb.ctypes.data))
         return lab, memb
+
+
+def _nclasses(label, seeds):
+    """Number of relaxation classes: distinct seed labels plus 0 (gft_ift.cpp:8684-8720)."""
+    import torch
+    if seeds.numel() == 0:
+        return 1
+    lab = label.reshape(-1)[seeds.long()]
+    lab = lab[lab >= 0]
+    return int(torch.unique(torch.cat([lab, torch.zeros(1, dtype=lab.dtype, device=lab.device)])).numel())
 
 
 def oift_fmax_gpu_run(h_image, h_labels, xsize, ysize, zsize, per_class, seeds):

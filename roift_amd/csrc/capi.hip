@@ -111,7 +111,8 @@ struct Weights {
     WeightInfo info;
 };
 
-Weights weights_into_ctx(roift_ctx &c, const Dims &d, int nadj, const void *img, int idt, float P)
+Weights weights_into_ctx(roift_ctx &c, const Dims &d, int nadj, const void *img, int idt, float P,
+                         const GradModel *gm = nullptr)
 {
     const long long slots = (long long)((nadj - 1) / 2) * d.n;
     Weights w{};
@@ -119,13 +120,13 @@ Weights weights_into_ctx(roift_ctx &c, const Dims &d, int nadj, const void *img,
     w.wa = buf<uint16_t>(c, "pipe.wa", slots);
     w.wb = buf<uint16_t>(c, "pipe.wb", slots);
     try {
-        w.info = build_weights(c, d, nadj, img, idt, P, w.wa, w.wb, ROIFT_U16);
+        w.info = build_weights(c, d, nadj, img, idt, P, w.wa, w.wb, ROIFT_U16, gm);
     } catch (const Error &e) {
         if (e.code != ROIFT_EUNSUPPORTED) throw;
         w.dtype = ROIFT_U32;
         w.wa = buf<uint32_t>(c, "pipe.wa32", slots);
         w.wb = buf<uint32_t>(c, "pipe.wb32", slots);
-        w.info = build_weights(c, d, nadj, img, idt, P, w.wa, w.wb, ROIFT_U32);
+        w.info = build_weights(c, d, nadj, img, idt, P, w.wa, w.wb, ROIFT_U32, gm);
     }
     ROIFT_REQUIRE(w.info.minI >= 0, ROIFT_EINVAL, "image has negative values (shift it to >= 0 first)");
     return w;
@@ -278,8 +279,105 @@ ROIFT_API int roift_relax_dev(roift_ctx *ctx, const roift_problem *pb, const voi
         const int nadj = nadj_of(pb);
         const RelaxSpec rs = relax_spec_of(pb);
         ev_record(*ctx, 2);
+        if (n_seeds == 0) {   // ORelax_1_Multi with an empty S runs ORelax_1 (gft_ift.cpp:8678-8682)
+            ROIFT_REQUIRE(!pb->use_per_class, ROIFT_EINVAL, "the per-class relaxation needs seeds");
+            run_relax_binary(*ctx, d, nadj, d_image, image_dtype, pb->per, nullptr, nullptr, nullptr, nullptr, nullptr,
+                             0, d_label, niter, d_memb);
+            if (h_nclasses) *h_nclasses = 1;
+        } else {
+            const WeightInfo wi = image_extrema(*ctx, d, d_image, image_dtype);
+            run_relax(*ctx, d, nadj, d_image, image_dtype, rs, d_seeds, n_seeds, d_label, niter, d_memb, h_nclasses,
+                      wi);
+        }
+        ev_record(*ctx, 3);
+        ctx->timings[2] = ev_ms(*ctx, 2, 3);
+        return ROIFT_OK;
+    });
+}
+
+ROIFT_API int roift_relax_binary_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                     const int32_t *d_W, const int32_t *d_Wx, const int32_t *d_Wy,
+                                     const int32_t *d_Wz, const int32_t *d_seeds, int n_seeds, int32_t *d_label,
+                                     int niter, float *d_memb)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        const int nadj = d_Wx ? 7 : nadj_of(pb);
+        ROIFT_REQUIRE(image_dtype == ROIFT_I16 || image_dtype == ROIFT_I32, ROIFT_EINVAL, "image dtype must be I16/I32");
+        ev_record(*ctx, 2);
+        run_relax_binary(*ctx, d, nadj, d_image, image_dtype, pb->per, d_W, d_Wx, d_Wy, d_Wz, d_seeds, n_seeds,
+                         d_label, niter, d_memb);
+        ev_record(*ctx, 3);
+        ctx->timings[2] = ev_ms(*ctx, 2, 3);
+        return ROIFT_OK;
+    });
+}
+
+ROIFT_API int roift_oift_gradweight_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                        const int32_t *d_grad, float alpha, const int32_t *d_seeds, int n_seeds,
+                                        int32_t *d_label, int32_t *d_cost)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        const int nadj = nadj_of(pb);
+        ROIFT_REQUIRE(!pb->use_per_class, ROIFT_EINVAL, "OIFT_Multi_GradWeight takes a single polarity");
+        ROIFT_REQUIRE(d_grad != nullptr, ROIFT_EINVAL, "null gradient image");
+        const Polarity po = polarity_of(pb);
+        GradModel gm;
+        gm.G = d_grad;
+        gm.alpha = alpha;
+        ev_record(*ctx, 0);
+        const Weights w = weights_into_ctx(*ctx, d, nadj, d_image, image_dtype, po.P, &gm);
+        ev_record(*ctx, 1);
+        const int lv = run_oift(*ctx, d, nadj, w.wa, w.wb, w.dtype, po.lc, d_seeds, n_seeds, d_label, d_cost);
+        ev_record(*ctx, 2);
+        ctx->timings[0] = ev_ms(*ctx, 0, 1);
+        ctx->timings[1] = ev_ms(*ctx, 1, 2);
+        return lv >= 0 ? ROIFT_OK : lv;
+    });
+}
+
+ROIFT_API int roift_relax_gradweight_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image,
+                                         int image_dtype, const int32_t *d_grad, float alpha, const int32_t *d_seeds,
+                                         int n_seeds, int32_t *d_label, int niter, float *d_memb)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        const int nadj = nadj_of(pb);
+        ROIFT_REQUIRE(!pb->use_per_class, ROIFT_EINVAL, "ORelax_1_Multi_GradWeight takes a single polarity");
+        ROIFT_REQUIRE(d_grad != nullptr, ROIFT_EINVAL, "null gradient image");
+        if (n_seeds == 0) return ROIFT_OK;   // exp_gradient_weight.cpp:296
+        const RelaxSpec rs = relax_spec_of(pb);
+        RelaxModel rm;
+        rm.model = RELAX_GRAD;
+        rm.G = d_grad;
+        rm.alpha = alpha;
+        ev_record(*ctx, 2);
         const WeightInfo wi = image_extrema(*ctx, d, d_image, image_dtype);
-        run_relax(*ctx, d, nadj, d_image, image_dtype, rs, d_seeds, n_seeds, d_label, niter, d_memb, h_nclasses, wi);
+        run_relax(*ctx, d, nadj, d_image, image_dtype, rs, d_seeds, n_seeds, d_label, niter, d_memb, nullptr, wi, rm);
+        ev_record(*ctx, 3);
+        ctx->timings[2] = ev_ms(*ctx, 2, 3);
+        return ROIFT_OK;
+    });
+}
+
+ROIFT_API int roift_relax_rbf_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                  float sigma, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter,
+                                  float *d_memb, float *h_sigma)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        const int nadj = nadj_of(pb);
+        ROIFT_REQUIRE(!pb->use_per_class, ROIFT_EINVAL, "the RBF relaxation has no per-class form");
+        RelaxModel rm;
+        rm.model = RELAX_RBF;
+        rm.sigma = sigma > 0.0f ? sigma : image_sigma(*ctx, d, d_image, image_dtype);
+        if (h_sigma) *h_sigma = rm.sigma;
+        if (n_seeds == 0) return ROIFT_OK;   // exp_gaussian_rbf_relax.cpp:235
+        const RelaxSpec rs = relax_spec_of(pb);
+        ev_record(*ctx, 2);
+        const WeightInfo wi = image_extrema(*ctx, d, d_image, image_dtype);
+        run_relax(*ctx, d, nadj, d_image, image_dtype, rs, d_seeds, n_seeds, d_label, niter, d_memb, nullptr, wi, rm);
         ev_record(*ctx, 3);
         ctx->timings[2] = ev_ms(*ctx, 2, 3);
         return ROIFT_OK;
@@ -351,6 +449,35 @@ ROIFT_API int roift_relax(roift_ctx *ctx, const roift_problem *pb, const int32_t
         ROIFT_HIP_CHECK(hipMemcpyAsync(h_label, lab, 4 * d.n, hipMemcpyDeviceToHost, ctx->stream));
         if (h_memb)
             ROIFT_HIP_CHECK(hipMemcpyAsync(h_memb, memb, 4 * d.n * (size_t)nc, hipMemcpyDeviceToHost, ctx->stream));
+        ROIFT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return ROIFT_OK;
+    });
+}
+
+ROIFT_API int roift_relax_binary(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image, const int32_t *h_W,
+                                 const int32_t *h_Wx, const int32_t *h_Wy, const int32_t *h_Wz,
+                                 const int32_t *h_seeds, int n_seeds, int32_t *h_label, int niter, float *h_memb)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        ROIFT_REQUIRE(n_seeds >= 0, ROIFT_EINVAL, "negative seed count");
+        auto up = [&](const int32_t *h, const char *name) -> int32_t * {
+            if (!h) return nullptr;
+            int32_t *p = buf<int32_t>(*ctx, name, d.n);
+            ROIFT_HIP_CHECK(hipMemcpyAsync(p, h, 4 * d.n, hipMemcpyHostToDevice, ctx->stream));
+            return p;
+        };
+        int32_t *img = up(h_image, "host.img");
+        int32_t *lab = up(h_label, "host.lab");
+        int32_t *W = up(h_W, "host.W"), *Wx = up(h_Wx, "host.Wx"), *Wy = up(h_Wy, "host.Wy"), *Wz = up(h_Wz, "host.Wz");
+        int32_t *S = buf<int32_t>(*ctx, "host.seeds", n_seeds > 0 ? n_seeds : 1);
+        if (n_seeds > 0)
+            ROIFT_HIP_CHECK(hipMemcpyAsync(S, h_seeds, 4 * (size_t)n_seeds, hipMemcpyHostToDevice, ctx->stream));
+        float *memb = h_memb ? buf<float>(*ctx, "host.memb", d.n) : nullptr;
+        int rc = roift_relax_binary_dev(ctx, pb, img, ROIFT_I32, W, Wx, Wy, Wz, S, n_seeds, lab, niter, memb);
+        if (rc != ROIFT_OK) return rc;
+        ROIFT_HIP_CHECK(hipMemcpyAsync(h_label, lab, 4 * d.n, hipMemcpyDeviceToHost, ctx->stream));
+        if (h_memb) ROIFT_HIP_CHECK(hipMemcpyAsync(h_memb, memb, 4 * d.n, hipMemcpyDeviceToHost, ctx->stream));
         ROIFT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return ROIFT_OK;
     });

@@ -77,9 +77,17 @@ struct WeightInfo {
 };
 
 // Entry points implemented in the .hip translation units.
+// Optional gradient-weight model of the builder (OIFT_Multi_GradWeight): int32 G, blend alpha.
+struct GradModel {
+    const int32_t *G = nullptr;
+    float alpha = 0.5f;
+};
+
 WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,
-                         void *d_wa, void *d_wb, int weight_dtype);
+                         void *d_wa, void *d_wb, int weight_dtype, const GradModel *gm = nullptr);
 WeightInfo image_extrema(roift_ctx &c, const Dims &d, const void *d_image, int image_dtype);
+// Global standard deviation of the image (estimate_sigma, exp_gaussian_rbf_relax.cpp:194-208).
+float image_sigma(roift_ctx &c, const Dims &d, const void *d_image, int image_dtype);
 
 // Class of each label for the OIFT: 1 = w_obj, 0 = w_bg (transposed).  Labels
 // outside [0, max_label] use default_cls.
@@ -98,8 +106,23 @@ struct RelaxSpec {
     int max_label = -1;
 };
 
+// Relaxation weight models: ORelax_1_Multi[_PerClass], ORelax_1_Multi_GradWeight, ORelax_1_Multi_GaussRBF.
+enum : int { RELAX_INTENSITY = 0, RELAX_GRAD = 1, RELAX_RBF = 2 };
+struct RelaxModel {
+    int model = RELAX_INTENSITY;
+    const int32_t *G = nullptr;   // RELAX_GRAD: normalised gradient image (int32, device)
+    float alpha = 0.5f;           // RELAX_GRAD: blend
+    float sigma = 1.0f;           // RELAX_RBF
+};
+
 int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, const RelaxSpec &rs,
               const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter, float *d_memb, int32_t *nclasses_out,
-              const WeightInfo &wi);
+              const WeightInfo &wi, const RelaxModel &rm = RelaxModel());
+
+// ORelax_1 (binary fallback): single channel f = label/Imax, hard polarity.  Weight source:
+// the image (W == Wx == nullptr), a per-voxel weight W, or per-axis Wx/Wy/Wz (6-adj only).
+int run_relax_binary(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,
+                     const int32_t *W, const int32_t *Wx, const int32_t *Wy, const int32_t *Wz,
+                     const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter, float *d_memb);
 
 }  // namespace roift

@@ -86,9 +86,9 @@ __attribute__((visibility("default"))) void OIFT_Multi_PerClass(sAdjRel3 *A, sSc
 __attribute__((visibility("default"))) void ORelax_1_Multi(sAdjRel3 *A, sScene32 *scn, float per, int *S,
                                                            sScene32 *label, int ntimes)
 {
-    if (S == nullptr || S[0] <= 0) {
-        std::fprintf(stderr, "roift: ORelax_1 (the empty-seed fallback) is not implemented on the GPU\n");
-        std::exit(-1);
+    if (S == nullptr || S[0] <= 0) {   // the binary fallback (gft_ift.cpp:8678-8682)
+        ORelax_1(A, scn, per, S, label, ntimes);
+        return;
     }
     const std::vector<int> s = seeds_of(S, label);
     const roift_problem pb = problem(A, scn, 0, per, nullptr, -1);
@@ -102,6 +102,40 @@ __attribute__((visibility("default"))) void ORelax_1_Multi_PerClass(sAdjRel3 *A,
     const std::vector<int> s = seeds_of(S, label);
     const roift_problem pb = problem(A, scn, 1, 0.0f, per_class, max_label);
     check(roift_relax(context(), &pb, scn->data, s.data(), (int)s.size(), label->data, ntimes, nullptr));
+}
+
+namespace {
+
+// S == NULL reads as no seeds here (the reference dereferences S[0]).
+void relax_binary(sAdjRel3 *A, sScene32 *scn, const int *W, const int *Wx, const int *Wy, const int *Wz, float per,
+                  int *S, sScene32 *label, int ntimes)
+{
+    const std::vector<int> s = S ? std::vector<int>(S + 1, S + 1 + S[0]) : std::vector<int>();
+    const roift_problem pb = problem(A, scn, 0, per, nullptr, -1);
+    check(roift_relax_binary(context(), &pb, scn->data, W, Wx, Wy, Wz, s.data(), (int)s.size(), label->data, ntimes,
+                             nullptr));
+}
+
+}  // namespace
+
+__attribute__((visibility("default"))) void ORelax_1(sAdjRel3 *A, sScene32 *scn, float per, int *S, sScene32 *label,
+                                                     int ntimes)
+{
+    relax_binary(A, scn, nullptr, nullptr, nullptr, nullptr, per, S, label, ntimes);
+}
+
+__attribute__((visibility("default"))) void ORelax_1(sScene32 *W, sAdjRel3 *A, sScene32 *scn, float per, int *S,
+                                                     sScene32 *label, int ntimes)
+{
+    relax_binary(A, scn, W->data, nullptr, nullptr, nullptr, per, S, label, ntimes);
+}
+
+__attribute__((visibility("default"))) void ORelax_1(sScene32 *Wx, sScene32 *Wy, sScene32 *Wz, sScene32 *scn,
+                                                     float per, int *S, sScene32 *label, int ntimes)
+{
+    static gft::AdjRel3::sDisplacement3 d6[7];
+    static sAdjRel3 six{d6, 7};   // the per-axis form always uses Spheric(1.0) (gft_ift.cpp:9355)
+    relax_binary(&six, scn, nullptr, Wx->data, Wy->data, Wz->data, per, S, label, ntimes);
 }
 
 }  // namespace ift

@@ -38,9 +38,42 @@ struct RelaxArgs {
     int nc;
     int bg;
     int cls_label[NC_MAX];
+    // GradWeight model: base alpha|dI| + (1-alpha)(G_p+G_q)/2; RBF model: exp(-d^2/sigma2)
+    const int32_t *G;
+    float alpha;
+    float sigma2;
 };
 
-template <int NADJ, typename IT>
+// Weight of neighbour q in p's update under the three relaxation models
+// (ORelax_1_Multi gft_ift.cpp:8800-8816; ORelax_1_Multi_GradWeight
+// exp_gradient_weight.cpp:407-422; ORelax_1_Multi_GaussRBF exp_gaussian_rbf_relax.cpp:343-345),
+// in the reference's FP32 operation order.  s is the soft polarity of q.
+template <int NADJ, int WM>
+__device__ inline float model_weight(const RelaxArgs &a, int k, int Ip, int Iq, long long p, long long q, float s)
+{
+    float w;
+    if constexpr (WM == RELAX_RBF) {
+        const float diff = (float)(Ip - Iq);
+        return expf(-(diff * diff) / a.sigma2) / a.Dpq[k];
+    } else if constexpr (WM == RELAX_GRAD) {
+        const float gav = (float)(a.G[p] + a.G[q]) * 0.5f;
+        w = a.alpha * fabsf((float)(Ip - Iq)) + (1.0f - a.alpha) * gav;
+    } else {
+        w = fabsf((float)(Ip - Iq));
+    }
+    if (Ip > Iq) w *= (1.0f + s / 100.0f);
+    else if (Ip < Iq) w *= (1.0f - s / 100.0f);
+    if constexpr (WM == RELAX_GRAD) w = fmaxf(w, 0.0f);
+    w = a.Wmax - w;
+    if (w < 0.0f) w = 0.0f;
+    w = w * w;
+    w = w * w;
+    w = w * w;
+    if (NADJ != 7) w /= a.Dpq[k];
+    return w;
+}
+
+template <int NADJ, typename IT, int WM>
 __global__ __launch_bounds__(256) void k_relax2(const IT *__restrict__ img, RelaxArgs a, const float *__restrict__ fin,
                                                  float *__restrict__ fout, const uint8_t *__restrict__ seedcls,
                                                  int32_t *__restrict__ label_out)
@@ -64,18 +97,10 @@ __global__ __launch_bounds__(256) void k_relax2(const IT *__restrict__ img, Rela
                 const long long q = qx + (long long)qy * d.xs + (long long)qz * xy;
                 const int Iq = (int)img[q];
                 const float fq = fin[q];
-                float w = fabsf((float)(Ip - Iq));
                 // soft polarity from the neighbour's memberships (gft_ift.cpp:8800-8802, 9077-9079)
                 const float s = a.use_pc ? (a.pcm[a.bg] * fq + a.pcm[1 - a.bg] * (1.0f - fq))
                                          : a.per * (2.0f * fq - 1.0f);
-                if (Ip > Iq) w *= (1.0f + s / 100.0f);
-                else if (Ip < Iq) w *= (1.0f - s / 100.0f);
-                w = a.Wmax - w;
-                if (w < 0.0f) w = 0.0f;
-                w = w * w;
-                w = w * w;
-                w = w * w;
-                if (NADJ != 7) w /= a.Dpq[k];
+                const float w = model_weight<NADJ, WM>(a, k, Ip, Iq, p, q, s);
                 sw += w;
                 acc += w * fq;
             }
@@ -86,7 +111,7 @@ __global__ __launch_bounds__(256) void k_relax2(const IT *__restrict__ img, Rela
     }
 }
 
-template <int NADJ, typename IT>
+template <int NADJ, typename IT, int WM>
 __global__ __launch_bounds__(256) void k_relaxn(const IT *__restrict__ img, RelaxArgs a, const float *__restrict__ fin,
                                                  float *__restrict__ fout, const uint8_t *__restrict__ seedcls,
                                                  int32_t *__restrict__ label_out)
@@ -110,22 +135,15 @@ __global__ __launch_bounds__(256) void k_relaxn(const IT *__restrict__ img, Rela
                 if (!in_volume(d, qx, qy, qz)) continue;
                 const long long q = qx + (long long)qy * d.xs + (long long)qz * xy;
                 const int Iq = (int)img[q];
-                float w = fabsf((float)(Ip - Iq));
-                float s;
-                if (a.use_pc) {
-                    s = 0.0f;
-                    for (int c = 0; c < nc; c++) s += a.pcm[c] * fin[c * d.n + q];
-                } else {
-                    s = a.per * (2.0f * fin[a.bg * d.n + q] - 1.0f);
+                float s = 0.0f;
+                if (WM != RELAX_RBF) {
+                    if (a.use_pc) {
+                        for (int c = 0; c < nc; c++) s += a.pcm[c] * fin[c * d.n + q];
+                    } else {
+                        s = a.per * (2.0f * fin[a.bg * d.n + q] - 1.0f);
+                    }
                 }
-                if (Ip > Iq) w *= (1.0f + s / 100.0f);
-                else if (Ip < Iq) w *= (1.0f - s / 100.0f);
-                w = a.Wmax - w;
-                if (w < 0.0f) w = 0.0f;
-                w = w * w;
-                w = w * w;
-                w = w * w;
-                if (NADJ != 7) w /= a.Dpq[k];
+                const float w = model_weight<NADJ, WM>(a, k, Ip, Iq, p, q, s);
                 sw += w;
                 for (int c = 0; c < nc; c++) v[c] += w * fin[c * d.n + q];
             }
@@ -325,17 +343,142 @@ __global__ void k_pack_image(const IT *__restrict__ img, const uint8_t *__restri
         out[p] = (int16_t)((int)img[p] | (seedcls[p] != NOSEED ? 0x8000 : 0));
 }
 
-template <int NADJ, typename IT>
-void launch_iter(roift_ctx &c, const void *img, const RelaxArgs &a, const float *fin, float *fout, const uint8_t *seedcls,
-                 int32_t *label_out)
+template <int NADJ, typename IT, int WM>
+void launch_iter_m(roift_ctx &c, const void *img, const RelaxArgs &a, const float *fin, float *fout,
+                   const uint8_t *seedcls, int32_t *label_out)
 {
     const unsigned g = grid_for(a.d.n, 256, 1 << 16);
     if (a.nc == 2)
-        hipLaunchKernelGGL((k_relax2<NADJ, IT>), dim3(g), dim3(256), 0, c.stream, (const IT *)img, a, fin, fout, seedcls,
-                           label_out);
+        hipLaunchKernelGGL((k_relax2<NADJ, IT, WM>), dim3(g), dim3(256), 0, c.stream, (const IT *)img, a, fin, fout,
+                           seedcls, label_out);
     else
-        hipLaunchKernelGGL((k_relaxn<NADJ, IT>), dim3(g), dim3(256), 0, c.stream, (const IT *)img, a, fin, fout, seedcls,
-                           label_out);
+        hipLaunchKernelGGL((k_relaxn<NADJ, IT, WM>), dim3(g), dim3(256), 0, c.stream, (const IT *)img, a, fin, fout,
+                           seedcls, label_out);
+    ROIFT_HIP_CHECK(hipGetLastError());
+}
+
+template <int NADJ, typename IT>
+void launch_iter(roift_ctx &c, const void *img, const RelaxArgs &a, int model, const float *fin, float *fout,
+                 const uint8_t *seedcls, int32_t *label_out)
+{
+    if (model == RELAX_GRAD) launch_iter_m<NADJ, IT, RELAX_GRAD>(c, img, a, fin, fout, seedcls, label_out);
+    else if (model == RELAX_RBF) launch_iter_m<NADJ, IT, RELAX_RBF>(c, img, a, fin, fout, seedcls, label_out);
+    else launch_iter_m<NADJ, IT, RELAX_INTENSITY>(c, img, a, fin, fout, seedcls, label_out);
+}
+
+// ---------------------------------------------------------------- ORelax_1 (binary)
+// Single channel f = label/Imax; hard polarity from f(q) < 0.5; the arc factor is applied
+// in double and no clamp precedes the eighth power (gft_ift.cpp:8570-8596).  Seeds keep
+// their initial value (the reference resets them to label/Imax after every iteration).
+// Weight base: the image (|I_p-I_q|, divided by Dpq), a per-voxel weight W (W_p+W_q,
+// gft_ift.cpp:9189-9331) or the per-axis weights of the neighbour's axis (gft_ift.cpp:9333-9498).
+enum : int { BIN_IMAGE = 0, BIN_W = 1, BIN_WXYZ = 2 };
+
+struct BinArgs {
+    Dims d;
+    float per;
+    float Wmax;   // (float) of the reference's int Wmax
+    float Dpq[27];
+    const int32_t *W;
+    const int32_t *Wa[3];   // x, y, z
+};
+
+template <int NADJ, typename IT, int BM>
+__global__ __launch_bounds__(256) void k_relax_bin(const IT *__restrict__ img, BinArgs a, const float *__restrict__ fin,
+                                                    float *__restrict__ fout, const uint8_t *__restrict__ seed,
+                                                    int32_t *__restrict__ label_out)
+{
+    using A = Adjacency<NADJ>;
+    const Dims d = a.d;
+    const long long xy = (long long)d.xs * d.ys;
+    for (long long p = gtid(); p < d.n; p += gstride()) {
+        float v;
+        if (seed[p]) {
+            v = fin[p];
+        } else {
+            const int x = (int)(p % d.xs), y = (int)((p / d.xs) % d.ys), z = (int)(p / xy);
+            const int Ip = (int)img[p];
+            float sw = 0.0f, acc = 0.0f;
+#pragma unroll
+            for (int k = 1; k < NADJ; k++) {
+                const int qx = x + A::dx(k), qy = y + A::dy(k), qz = z + A::dz(k);
+                if (!in_volume(d, qx, qy, qz)) continue;
+                const long long q = qx + (long long)qy * d.xs + (long long)qz * xy;
+                const int Iq = (int)img[q];
+                const float fq = fin[q];
+                float w;
+                if constexpr (BM == BIN_WXYZ) {
+                    const int t = A::dz(k) != 0 ? 2 : (A::dy(k) != 0 ? 1 : 0);
+                    w = (float)(a.Wa[t][p] + a.Wa[t][q]);
+                } else if constexpr (BM == BIN_W) {
+                    w = (float)(a.W[p] + a.W[q]);
+                } else {
+                    w = (float)abs(Ip - Iq);
+                }
+                const float per_pq = (fq < 0.5f) ? a.per : -a.per;
+                if (Ip > Iq) w = (float)((double)w * (1.0 + (double)per_pq / 100.0));
+                else if (Ip < Iq) w = (float)((double)w * (1.0 - (double)per_pq / 100.0));
+                w = a.Wmax - w;
+                w = w * w;
+                w = w * w;
+                w = w * w;
+                if (BM == BIN_IMAGE && NADJ != 7) w /= a.Dpq[k];
+                sw += w;
+                acc += w * fq;
+            }
+            v = acc / sw;
+        }
+        fout[p] = v;
+        if (label_out) label_out[p] = (v < 0.5f) ? 0 : 1;
+    }
+}
+
+__global__ void k_label_max(const int32_t *__restrict__ label, long long n, int *__restrict__ out)
+{
+    int m = INT32_MIN;
+    for (long long p = gtid(); p < n; p += gstride()) m = max(m, label[p]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+    if (__lane_id() == 0) atomicMax(out, m);
+}
+
+__global__ void k_bin_init(const int32_t *__restrict__ label, long long n, float imax, float *__restrict__ f,
+                           uint8_t *__restrict__ seed)
+{
+    for (long long p = gtid(); p < n; p += gstride()) {
+        f[p] = (float)label[p] / imax;
+        seed[p] = 0;
+    }
+}
+
+__global__ void k_bin_seeds(const int32_t *__restrict__ S, int ns, long long n, uint8_t *__restrict__ seed,
+                            int *__restrict__ bad)
+{
+    for (long long i = gtid(); i < ns; i += gstride()) {
+        const int v = S[i];
+        if (v >= 0 && v < n) seed[v] = 1;
+        else atomicOr(bad, 1);
+    }
+}
+
+__global__ void k_bin_labels(const float *__restrict__ f, long long n, int32_t *__restrict__ label)
+{
+    for (long long p = gtid(); p < n; p += gstride()) label[p] = (f[p] < 0.5f) ? 0 : 1;
+}
+
+template <int NADJ, typename IT>
+void launch_bin(roift_ctx &c, int bm, const void *img, const BinArgs &a, const float *fin, float *fout,
+                const uint8_t *seed, int32_t *lo)
+{
+    const unsigned g = grid_for(a.d.n, 256, 1 << 16);
+    if (bm == BIN_WXYZ)
+        hipLaunchKernelGGL((k_relax_bin<NADJ, IT, BIN_WXYZ>), dim3(g), dim3(256), 0, c.stream, (const IT *)img, a, fin,
+                           fout, seed, lo);
+    else if (bm == BIN_W)
+        hipLaunchKernelGGL((k_relax_bin<NADJ, IT, BIN_W>), dim3(g), dim3(256), 0, c.stream, (const IT *)img, a, fin,
+                           fout, seed, lo);
+    else
+        hipLaunchKernelGGL((k_relax_bin<NADJ, IT, BIN_IMAGE>), dim3(g), dim3(256), 0, c.stream, (const IT *)img, a,
+                           fin, fout, seed, lo);
     ROIFT_HIP_CHECK(hipGetLastError());
 }
 
@@ -343,9 +486,12 @@ void launch_iter(roift_ctx &c, const void *img, const RelaxArgs &a, const float 
 
 int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, const RelaxSpec &rs,
               const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter, float *d_memb, int32_t *nclasses_out,
-              const WeightInfo &wi)
+              const WeightInfo &wi, const RelaxModel &rm)
 {
     const int32_t maxI = wi.maxI;
+    ROIFT_REQUIRE(rm.model == RELAX_INTENSITY || !rs.use_per_class, ROIFT_EINVAL,
+                  "the gradient-weight and RBF relaxations take a single polarity");
+    ROIFT_REQUIRE(rm.model != RELAX_GRAD || rm.G != nullptr, ROIFT_EINVAL, "gradient-weight relaxation needs G");
     ROIFT_REQUIRE(nadj == 7 || nadj == 27, ROIFT_EUNSUPPORTED, "adjacency must be Spheric(1.0) or Spheric(1.75)");
     ROIFT_REQUIRE(n_seeds > 0, ROIFT_EINVAL, "relaxation needs seeds (the S==NULL form is ORelax_1)");
     ROIFT_REQUIRE(niter >= 0, ROIFT_EINVAL, "negative iteration count");
@@ -398,8 +544,13 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
             max_abs = std::max(max_abs, std::fabs(a.pcm[k]));
         }
     }
-    // float Wmax = (float)Imax; Wmax *= (1.0 + |per|/100.0)   (gft_ift.cpp:8733-8734)
+    // float Wmax = (float)Imax; Wmax *= (1.0 + |per|/100.0)   (gft_ift.cpp:8733-8734);
+    // the GradWeight relaxation computes the same in float (exp_gradient_weight.cpp:341-342)
     a.Wmax = (float)((double)(float)maxI * (1.0 + (double)max_abs / 100.0));
+    if (rm.model == RELAX_GRAD) a.Wmax = (float)maxI * (1.0f + std::fabs(rs.per) / 100.0f);
+    a.G = rm.G;
+    a.alpha = rm.alpha;
+    a.sigma2 = rm.sigma * rm.sigma;
     for (int k = 0; k < 27; k++) a.Dpq[k] = 1.0f;
     if (nadj == 27)
         for (int k = 1; k < 27; k++) {
@@ -437,7 +588,8 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
     float *fin = f0, *fout = f1;
     const bool i16 = image_dtype == ROIFT_I16;
     // z-marching form: 6-adjacency, two classes, intensities in [0, 32767] (bit 15 = seed)
-    const bool march = nadj == 7 && nc == 2 && wi.minI >= 0 && wi.maxI <= 0x7FFF && n < (1ll << 31) && niter > 0;
+    const bool march = rm.model == RELAX_INTENSITY && nadj == 7 && nc == 2 && wi.minI >= 0 && wi.maxI <= 0x7FFF &&
+                       n < (1ll << 31) && niter > 0;
     int16_t *pimg = nullptr;
     if (march) {
         pimg = buf<int16_t>(c, "relax.pimg", n);
@@ -455,11 +607,11 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
             hipLaunchKernelGGL(k_relax2_march, mgrid, dim3(MX, MY), 0, st, pimg, a, fin, fout, lo);
             ROIFT_HIP_CHECK(hipGetLastError());
         } else if (nadj == 7) {
-            if (i16) launch_iter<7, int16_t>(c, d_image, a, fin, fout, seedcls, lo);
-            else launch_iter<7, int32_t>(c, d_image, a, fin, fout, seedcls, lo);
+            if (i16) launch_iter<7, int16_t>(c, d_image, a, rm.model, fin, fout, seedcls, lo);
+            else launch_iter<7, int32_t>(c, d_image, a, rm.model, fin, fout, seedcls, lo);
         } else {
-            if (i16) launch_iter<27, int16_t>(c, d_image, a, fin, fout, seedcls, lo);
-            else launch_iter<27, int32_t>(c, d_image, a, fin, fout, seedcls, lo);
+            if (i16) launch_iter<27, int16_t>(c, d_image, a, rm.model, fin, fout, seedcls, lo);
+            else launch_iter<27, int32_t>(c, d_image, a, rm.model, fin, fout, seedcls, lo);
         }
         std::swap(fin, fout);
     }
@@ -472,6 +624,95 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
         ROIFT_HIP_CHECK(hipGetLastError());
     }
     ROIFT_HIP_CHECK(hipStreamSynchronize(st));
+    c.ktimes[2] = ev_ms(c, 12, 13);
+    c.ktimes[3] = niter;
+    return 0;
+}
+
+}  // namespace roift
+
+namespace roift {
+
+int run_relax_binary(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,
+                     const int32_t *W, const int32_t *Wx, const int32_t *Wy, const int32_t *Wz,
+                     const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter, float *d_memb)
+{
+    ROIFT_REQUIRE(nadj == 7 || nadj == 27, ROIFT_EUNSUPPORTED, "adjacency must be Spheric(1.0) or Spheric(1.75)");
+    ROIFT_REQUIRE(niter >= 0 && n_seeds >= 0, ROIFT_EINVAL, "negative iteration or seed count");
+    const bool wxyz = Wx != nullptr;
+    ROIFT_REQUIRE(!wxyz || (Wy && Wz), ROIFT_EINVAL, "per-axis weights need Wx, Wy and Wz");
+    ROIFT_REQUIRE(!wxyz || !W, ROIFT_EINVAL, "give either W or Wx/Wy/Wz");
+    // the per-axis form always runs on Spheric(1.0) (gft_ift.cpp:9355)
+    if (wxyz) nadj = 7;
+    const long long n = d.n;
+    hipStream_t st = c.stream;
+    const int bm = wxyz ? BIN_WXYZ : (W ? BIN_W : BIN_IMAGE);
+    // Wmax (int in the reference): max I, or 2 max W, or 2 max(max Wx, max Wy, max Wz), times (1+|per|/100)
+    int base_max;
+    if (bm == BIN_IMAGE) {
+        base_max = image_extrema(c, d, d_image, image_dtype).maxI;
+    } else if (bm == BIN_W) {
+        base_max = image_extrema(c, d, W, ROIFT_I32).maxI * 2;
+    } else {
+        base_max = std::max(image_extrema(c, d, Wx, ROIFT_I32).maxI,
+                            std::max(image_extrema(c, d, Wy, ROIFT_I32).maxI, image_extrema(c, d, Wz, ROIFT_I32).maxI)) *
+                   2;
+    }
+    BinArgs a{};
+    a.d = d;
+    a.per = per;
+    a.Wmax = (float)(int)((double)base_max * (1.0 + (double)std::fabs(per) / 100.0));
+    for (int k = 0; k < 27; k++) a.Dpq[k] = 1.0f;
+    if (nadj == 27)
+        for (int k = 1; k < 27; k++) {
+            const int ax = Adjacency<27>::dx(k), ay = Adjacency<27>::dy(k), az = Adjacency<27>::dz(k);
+            a.Dpq[k] = sqrtf((float)(ax * ax) * 1.0f * 1.0f + (float)(ay * ay) * 1.0f * 1.0f +
+                             (float)(az * az) * 1.0f * 1.0f) / 1.0f;
+        }
+    a.W = W;
+    a.Wa[0] = Wx;
+    a.Wa[1] = Wy;
+    a.Wa[2] = Wz;
+    // Imax = max label (gft_ift.cpp:8525)
+    int *info = buf<int>(c, "bin.info", 2);
+    const int info0[2] = {INT32_MIN, 0};
+    ROIFT_HIP_CHECK(hipMemcpyAsync(info, info0, sizeof(info0), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_label_max, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, d_label, n, info);
+    ROIFT_HIP_CHECK(hipGetLastError());
+    int hinfo[2];
+    ROIFT_HIP_CHECK(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
+    ROIFT_HIP_CHECK(hipStreamSynchronize(st));
+    float *f0 = buf<float>(c, "relax.f0", n);
+    float *f1 = buf<float>(c, "relax.f1", n);
+    uint8_t *seed = buf<uint8_t>(c, "relax.seedcls", n);
+    const unsigned G = grid_for(n, 256, 8192);
+    hipLaunchKernelGGL(k_bin_init, dim3(G), dim3(256), 0, st, d_label, n, (float)hinfo[0], f0, seed);
+    if (n_seeds > 0)
+        hipLaunchKernelGGL(k_bin_seeds, dim3(grid_for(n_seeds)), dim3(256), 0, st, d_seeds, n_seeds, n, seed, info + 1);
+    ROIFT_HIP_CHECK(hipGetLastError());
+    const bool i16 = image_dtype == ROIFT_I16;
+    float *fin = f0, *fout = f1;
+    ev_record(c, 12);
+    for (int it = 0; it < niter; it++) {
+        int32_t *lo = (it == niter - 1) ? d_label : nullptr;
+        if (nadj == 7) {
+            if (i16) launch_bin<7, int16_t>(c, bm, d_image, a, fin, fout, seed, lo);
+            else launch_bin<7, int32_t>(c, bm, d_image, a, fin, fout, seed, lo);
+        } else {
+            if (i16) launch_bin<27, int16_t>(c, bm, d_image, a, fin, fout, seed, lo);
+            else launch_bin<27, int32_t>(c, bm, d_image, a, fin, fout, seed, lo);
+        }
+        std::swap(fin, fout);
+    }
+    ev_record(c, 13);
+    if (niter == 0) {
+        hipLaunchKernelGGL(k_bin_labels, dim3(G), dim3(256), 0, st, fin, n, d_label);
+        ROIFT_HIP_CHECK(hipGetLastError());
+    }
+    if (d_memb) ROIFT_HIP_CHECK(hipMemcpyAsync(d_memb, fin, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    ROIFT_HIP_CHECK(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
+    ROIFT_HIP_CHECK(hipStreamSynchronize(st));
+    ROIFT_REQUIRE(hinfo[1] == 0, ROIFT_EINVAL, "seed index out of range");
     c.ktimes[2] = ev_ms(c, 12, 13);
     c.ktimes[3] = niter;
     return 0;

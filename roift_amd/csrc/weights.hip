@@ -17,10 +17,20 @@ namespace roift {
 // Block shape of the weight builder: 64 x 4 (x, y) voxels, marching WZ planes in z.
 constexpr int WX = 64, WY = 4, WZ = 16;
 
-template <int NADJ, typename IT, typename WT>
+// GRAD = false: OIFT_Multi's weight, |I_p-I_q| * (1 +- per/100) in double, truncated.
+// GRAD = true:  OIFT_Multi_GradWeight's weight (exp_gradient_weight.cpp:244-254), all in
+// float: wf = alpha|I_p-I_q| + (1-alpha)(G_p+G_q)*0.5, (int)(wf (1 +- per/100)), (int)wf when
+// I_p == I_q, clamped at 0.  wf is symmetric in p and q, so the two-values-per-edge
+// layout holds for it too.  This file is compiled without FMA contraction.
+struct GradArgs {
+    const int32_t *G;
+    float alpha, fpf, fmf;
+};
+
+template <int NADJ, typename IT, typename WT, bool GRAD>
 __global__ __launch_bounds__(WX *WY) void k_weights(const IT *__restrict__ img, Dims d, double fp, double fm,
-                                                   WT *__restrict__ wa, WT *__restrict__ wb, int *__restrict__ ext,
-                                                   uint32_t *__restrict__ overflow)
+                                                   GradArgs ga, WT *__restrict__ wa, WT *__restrict__ wb,
+                                                   int *__restrict__ ext, uint32_t *__restrict__ overflow)
 {
     using A = Adjacency<NADJ>;
     constexpr uint32_t WMAXV = sizeof(WT) == 2 ? 0xFFFFu : 0xFFFFFFFFu;
@@ -44,10 +54,21 @@ __global__ __launch_bounds__(WX *WY) void k_weights(const IT *__restrict__ img, 
                     const long long q = p + A::dx(i) + (long long)A::dy(i) * d.xs + (long long)A::dz(i) * xy;
                     const int Iq = (int)img[q];
                     const int dd = abs(Ip - Iq);
+                    int up, dn, eq;
+                    if constexpr (GRAD) {
+                        const float gav = (float)(ga.G[p] + ga.G[q]) * 0.5f;
+                        const float wf = ga.alpha * (float)dd + (1.0f - ga.alpha) * gav;
+                        eq = max(0, (int)wf);
+                        up = max(0, (int)(wf * ga.fpf));
+                        dn = max(0, (int)(wf * ga.fmf));
+                    } else {
+                        eq = dd;
+                        up = (int)((double)dd * fp);
+                        dn = (int)((double)dd * fm);
+                    }
                     // w_obj(p->q): per_pq=+per; w_obj(q->p): roles of p and q swapped
-                    const int up = (int)((double)dd * fp), dn = (int)((double)dd * fm);
-                    a = (uint32_t)(Ip > Iq ? up : (Ip < Iq ? dn : dd));
-                    b = (uint32_t)(Iq > Ip ? up : (Iq < Ip ? dn : dd));
+                    a = (uint32_t)(Ip > Iq ? up : (Ip < Iq ? dn : eq));
+                    b = (uint32_t)(Iq > Ip ? up : (Iq < Ip ? dn : eq));
                     if (a >= WMAXV || b >= WMAXV) ovf = 1;
                 }
                 wa[s * d.n + p] = (WT)a;
@@ -96,24 +117,49 @@ __global__ __launch_bounds__(256) void k_extrema(const IT *__restrict__ img, lon
     }
 }
 
+// Integer moments for estimate_sigma (exp_gaussian_rbf_relax.cpp:194-208): exact 64-bit sums.
+template <typename IT>
+__global__ __launch_bounds__(256) void k_moments(const IT *__restrict__ img, long long n,
+                                                 unsigned long long *__restrict__ out)
+{
+    long long s1 = 0, s2 = 0;
+    for (long long p = gtid(); p < n; p += gstride()) {
+        const long long v = (long long)img[p];
+        s1 += v;
+        s2 += v * v;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s1 += __shfl_xor(s1, o);
+        s2 += __shfl_xor(s2, o);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(&out[0], (unsigned long long)s1);
+        atomicAdd(&out[1], (unsigned long long)s2);
+    }
+}
+
 namespace {
 
 template <int NADJ, typename IT, typename WT>
-void launch_weights(roift_ctx &c, const Dims &d, const void *img, double fp, double fm, void *wa, void *wb, int *ext,
-                    uint32_t *ovf)
+void launch_weights(roift_ctx &c, const Dims &d, const void *img, double fp, double fm, const GradArgs &ga, void *wa,
+                    void *wb, int *ext, uint32_t *ovf)
 {
     ROIFT_REQUIRE((d.ys + WY - 1) / WY <= 65535 && (d.zs + WZ - 1) / WZ <= 65535, ROIFT_EUNSUPPORTED,
                   "volume too large for the weight builder's grid");
     const dim3 grid((d.xs + WX - 1) / WX, (d.ys + WY - 1) / WY, (d.zs + WZ - 1) / WZ);
-    hipLaunchKernelGGL((k_weights<NADJ, IT, WT>), grid, dim3(WX, WY), 0, c.stream, (const IT *)img, d, fp, fm, (WT *)wa,
-                       (WT *)wb, ext, ovf);
+    if (ga.G)
+        hipLaunchKernelGGL((k_weights<NADJ, IT, WT, true>), grid, dim3(WX, WY), 0, c.stream, (const IT *)img, d, fp,
+                           fm, ga, (WT *)wa, (WT *)wb, ext, ovf);
+    else
+        hipLaunchKernelGGL((k_weights<NADJ, IT, WT, false>), grid, dim3(WX, WY), 0, c.stream, (const IT *)img, d, fp,
+                           fm, ga, (WT *)wa, (WT *)wb, ext, ovf);
     ROIFT_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace
 
 WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,
-                         void *d_wa, void *d_wb, int weight_dtype)
+                         void *d_wa, void *d_wb, int weight_dtype, const GradModel *gm)
 {
     ROIFT_REQUIRE(nadj == 7 || nadj == 27, ROIFT_EUNSUPPORTED, "adjacency must be Spheric(1.0) or Spheric(1.75)");
     ROIFT_REQUIRE(image_dtype == ROIFT_I16 || image_dtype == ROIFT_I32, ROIFT_EINVAL, "image dtype must be I16 or I32");
@@ -124,18 +170,25 @@ WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_im
     ROIFT_HIP_CHECK(hipMemcpyAsync(ext, init, sizeof(init), hipMemcpyHostToDevice, c.stream));
     // per_pq / 100.0 in double, as the reference's "w *= (1.0 +- per_pq/100.0)"
     const double fp = 1.0 + (double)per / 100.0, fm = 1.0 - (double)per / 100.0;
+    GradArgs ga{nullptr, 0.0f, 0.0f, 0.0f};
+    if (gm && gm->G) {   // (1.0f +- per_pq/100.0f) in float, as the GradWeight experiment
+        ga.G = gm->G;
+        ga.alpha = gm->alpha;
+        ga.fpf = 1.0f + per / 100.0f;
+        ga.fmf = 1.0f - per / 100.0f;
+    }
     const bool i16 = image_dtype == ROIFT_I16, w16 = weight_dtype == ROIFT_U16;
     ev_record(c, 8);
     if (nadj == 7) {
-        if (i16 && w16) launch_weights<7, int16_t, uint16_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
-        else if (i16) launch_weights<7, int16_t, uint32_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
-        else if (w16) launch_weights<7, int32_t, uint16_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
-        else launch_weights<7, int32_t, uint32_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
+        if (i16 && w16) launch_weights<7, int16_t, uint16_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
+        else if (i16) launch_weights<7, int16_t, uint32_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
+        else if (w16) launch_weights<7, int32_t, uint16_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
+        else launch_weights<7, int32_t, uint32_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
     } else {
-        if (i16 && w16) launch_weights<27, int16_t, uint16_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
-        else if (i16) launch_weights<27, int16_t, uint32_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
-        else if (w16) launch_weights<27, int32_t, uint16_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
-        else launch_weights<27, int32_t, uint32_t>(c, d, d_image, fp, fm, d_wa, d_wb, ext, ovf);
+        if (i16 && w16) launch_weights<27, int16_t, uint16_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
+        else if (i16) launch_weights<27, int16_t, uint32_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
+        else if (w16) launch_weights<27, int32_t, uint16_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
+        else launch_weights<27, int32_t, uint32_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
     }
     ev_record(c, 9);
     int host[4];
@@ -162,6 +215,31 @@ WeightInfo image_extrema(roift_ctx &c, const Dims &d, const void *d_image, int i
     ROIFT_HIP_CHECK(hipMemcpyAsync(host, ext, sizeof(host), hipMemcpyDeviceToHost, c.stream));
     ROIFT_HIP_CHECK(hipStreamSynchronize(c.stream));
     return WeightInfo{host[0], host[1]};
+}
+
+}  // namespace roift
+
+namespace roift {
+
+float image_sigma(roift_ctx &c, const Dims &d, const void *d_image, int image_dtype)
+{
+    unsigned long long *m = buf<unsigned long long>(c, "w.moments", 2);
+    const unsigned long long zero[2] = {0, 0};
+    ROIFT_HIP_CHECK(hipMemcpyAsync(m, zero, sizeof(zero), hipMemcpyHostToDevice, c.stream));
+    if (image_dtype == ROIFT_I16)
+        hipLaunchKernelGGL(k_moments<int16_t>, dim3(grid_for(d.n, 256, 4096)), dim3(256), 0, c.stream,
+                           (const int16_t *)d_image, d.n, m);
+    else
+        hipLaunchKernelGGL(k_moments<int32_t>, dim3(grid_for(d.n, 256, 4096)), dim3(256), 0, c.stream,
+                           (const int32_t *)d_image, d.n, m);
+    ROIFT_HIP_CHECK(hipGetLastError());
+    unsigned long long h[2];
+    ROIFT_HIP_CHECK(hipMemcpyAsync(h, m, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    ROIFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+    const double mean = (double)(long long)h[0] / (double)d.n;
+    double var = (double)(long long)h[1] / (double)d.n - mean * mean;
+    if (var < 1.0) var = 1.0;
+    return (float)std::sqrt(var);
 }
 
 }  // namespace roift

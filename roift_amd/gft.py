@@ -5,6 +5,9 @@ Same names, argument meaning and in-place behaviour as the C++ originals:
   gft::AdjRel3::Spheric(r)                     (gft_adjrel3.cpp:47-85)
   gft::ift::OIFT_Multi / OIFT_Multi_PerClass   (gft_ift.h:108-122)
   gft::ift::ORelax_1_Multi / _PerClass         (gft_ift.h:451-465)
+  gft::ift::ORelax_1 (image / W / Wx,Wy,Wz)     (gft_ift.h:443-482)
+  and, in `exp`, the experiments' OIFT_Multi_GradWeight, ORelax_1_Multi_GradWeight,
+  ORelax_1_Multi_GaussRBF and estimate_sigma
 The gft seed array convention is kept: S[0] = count, S[1..S[0]] voxel indices.
 Where the reference calls exit(-1) this raises RoiftError.
 """
@@ -14,7 +17,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import Context, RoiftError, EUNSUPPORTED, NIL
+from . import Context, NIL
 
 _ctx = None
 
@@ -121,9 +124,29 @@ class ift:
 
     @staticmethod
     def ORelax_1_Multi(A, scn, per, S, label, ntimes) -> None:
-        if S is None or int(np.asarray(S)[0]) <= 0:
-            raise RoiftError(EUNSUPPORTED, "ORelax_1 (single-channel fallback for empty S) is not implemented")
+        if S is None or int(np.asarray(S)[0]) <= 0:   # binary fallback (gft_ift.cpp:8678-8682)
+            return ift.ORelax_1(A, scn, per, S, label, ntimes)
         lab, _ = _context().relax_host(scn.volume(), _seeds_of(S, label), label.volume(), ntimes, A.r, per)
+        label.data[:] = lab.ravel()
+
+    @staticmethod
+    def ORelax_1(*args) -> None:
+        """ORelax_1(A, scn, per, S, label, ntimes), ORelax_1(W, A, scn, per, S, label, ntimes) or
+        ORelax_1(Wx, Wy, Wz, scn, per, S, label, ntimes) (gft_ift.h:443-482)."""
+        W = Wxyz = None
+        if len(args) == 6:
+            A, scn, per, S, label, ntimes = args
+        elif len(args) == 7:
+            W, A, scn, per, S, label, ntimes = args
+            W = W.volume()
+        elif len(args) == 8:
+            wx, wy, wz, scn, per, S, label, ntimes = args
+            Wxyz = (wx.volume(), wy.volume(), wz.volume())
+            A = AdjRel3.Spheric(1.0)
+        else:
+            raise TypeError("ORelax_1 takes 6, 7 or 8 arguments")
+        seeds = None if S is None else np.asarray(S, np.int32)[1:1 + int(np.asarray(S)[0])]
+        lab, _ = _context().relax_binary_host(scn.volume(), seeds, label.volume(), ntimes, A.r, per, W, Wxyz)
         label.data[:] = lab.ravel()
 
     @staticmethod
@@ -133,3 +156,58 @@ class ift:
         pc = np.asarray(per_class, np.float32)[: max_label + 1]
         lab, _ = _context().relax_host(scn.volume(), _seeds_of(S, label), label.volume(), ntimes, A.r, 0.0, pc)
         label.data[:] = lab.ravel()
+
+
+class exp:
+    """The experiment variants of the hot path (experiments/exp_gradient_weight.cpp,
+    experiments/exp_gaussian_rbf_relax.cpp), with the experiments' argument order."""
+
+    @staticmethod
+    def OIFT_Multi_GradWeight(A, scn, grad_norm, per, alpha, S, label) -> None:
+        import torch
+        ctx = _context()
+        dev = torch.device("cuda", ctx.device)
+        img = torch.from_numpy(scn.volume()).to(dev)
+        g = torch.from_numpy(grad_norm.volume()).to(dev)
+        lab = torch.from_numpy(label.volume().copy()).to(dev)
+        seeds = torch.from_numpy(_seeds_of(S, label)).to(dev)
+        ctx.oift_gradweight(img, g, seeds, lab, alpha, A.r, per, want_cost=False)
+        label.data[:] = lab.cpu().numpy().ravel()
+
+    @staticmethod
+    def ORelax_1_Multi_GradWeight(A, scn, grad_norm, per, alpha, S, label, ntimes) -> None:
+        import torch
+        if S is None or int(np.asarray(S)[0]) <= 0:
+            return   # exp_gradient_weight.cpp:296
+        ctx = _context()
+        dev = torch.device("cuda", ctx.device)
+        img = torch.from_numpy(scn.volume()).to(dev)
+        g = torch.from_numpy(grad_norm.volume()).to(dev)
+        lab = torch.from_numpy(label.volume().copy()).to(dev)
+        seeds = torch.from_numpy(_seeds_of(S, label)).to(dev)
+        ctx.relax_gradweight(img, g, seeds, lab, ntimes, alpha, A.r, per, want_memb=False)
+        label.data[:] = lab.cpu().numpy().ravel()
+
+    @staticmethod
+    def ORelax_1_Multi_GaussRBF(A, scn, per, sigma, S, label, ntimes) -> None:
+        import torch
+        if S is None or int(np.asarray(S)[0]) <= 0:
+            return   # exp_gaussian_rbf_relax.cpp:235
+        ctx = _context()
+        dev = torch.device("cuda", ctx.device)
+        img = torch.from_numpy(scn.volume()).to(dev)
+        lab = torch.from_numpy(label.volume().copy()).to(dev)
+        seeds = torch.from_numpy(_seeds_of(S, label)).to(dev)
+        ctx.relax_rbf(img, seeds, lab, ntimes, sigma, A.r, want_memb=False)
+        label.data[:] = lab.cpu().numpy().ravel()
+
+    @staticmethod
+    def estimate_sigma(scn) -> float:
+        """Global standard deviation (exp_gaussian_rbf_relax.cpp:194-208), computed on the device."""
+        import torch
+        ctx = _context()
+        img = torch.from_numpy(scn.volume()).to(torch.device("cuda", ctx.device))
+        lab = torch.zeros(img.shape, dtype=torch.int32, device=img.device)
+        seeds = torch.zeros(0, dtype=torch.int32, device=img.device)
+        _, used = ctx.relax_rbf(img, seeds, lab, 0, 0.0, want_memb=False)
+        return used

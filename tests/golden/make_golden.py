@@ -1,7 +1,8 @@
 """Writes the golden fixtures in this directory from the compiled reference.
 
 Runs gft::ift::OIFT_Multi[_PerClass] and ORelax_1_Multi[_PerClass] of the
-reference (oracle/_ref/libroift_ref.so, built by `make -C oracle ref` from the
+reference, and (into variants/) its ORelax_1 binary forms and the experiment
+variants OIFT_Multi_GradWeight / ORelax_1_Multi_GradWeight / ORelax_1_Multi_GaussRBF (oracle/_ref/libroift_ref.so, built by `make -C oracle ref` from the
 reference sources) on small deterministic inputs and stores inputs and
 outputs as .npz.  The reference holds no fixtures of its own, so these are
 the reference-produced known answers the oracle and the GPU path are pinned
@@ -58,6 +59,81 @@ def cases():
                            per_class=None, niter=3)
 
 
+def grad_image(img):
+    """Deterministic gradient-magnitude image scaled to [0, max I] as the GradWeight experiment
+    scales its Sobel magnitude (exp_gradient_weight.cpp:593-607); central differences here."""
+    I = img.astype(np.int64)
+    g = np.zeros_like(I)
+    for ax in range(3):
+        if I.shape[ax] > 2:
+            d = np.abs(np.roll(I, -1, ax) - np.roll(I, 1, ax))
+            g += d
+    gmax, imax = int(g.max()), int(I.max())
+    if gmax > 0 and imax > 0:
+        g = g * imax // gmax
+    return g.astype(np.int32)
+
+
+def variant_cases():
+    """Experiment variants and the ORelax_1 binary forms on small phantoms."""
+    for i, (shape, seed, adj, alpha) in enumerate([((12, 14, 16), 41, 1.0, 0.5), ((10, 12, 14), 42, 1.75, 0.3),
+                                                   ((14, 12, 10), 43, 1.0, 0.0)]):
+        c = P.small_case(shape, seed, per=50.0, top=255)
+        yield f"grad_{i}", dict(kind="grad", image=c.image, seeds=c.seeds, seed_labels=c.seed_labels, adj_r=adj,
+                                per=50.0, alpha=alpha, sigma=0.0, niter=6)
+    for i, (shape, seed, adj, sig) in enumerate([((12, 14, 16), 51, 1.0, 0.0), ((10, 12, 14), 52, 1.75, 0.0),
+                                                 ((12, 12, 12), 53, 1.0, 40.0)]):
+        c = P.small_case(shape, seed, per=50.0, top=255)
+        yield f"rbf_{i}", dict(kind="rbf", image=c.image, seeds=c.seeds, seed_labels=c.seed_labels, adj_r=adj,
+                               per=50.0, alpha=0.0, sigma=sig, niter=8)
+    for i, (shape, seed, adj, form) in enumerate([((12, 14, 16), 61, 1.0, "image"), ((10, 12, 14), 62, 1.75, "image"),
+                                                  ((12, 14, 16), 63, 1.0, "W"), ((12, 12, 14), 64, 1.0, "Wxyz"),
+                                                  ((12, 14, 16), 65, 1.0, "noseeds")]):
+        c = P.small_case(shape, seed, per=50.0, top=255)
+        yield f"binary_{i}", dict(kind="binary", form=form, image=c.image, seeds=c.seeds, seed_labels=c.seed_labels,
+                                  adj_r=adj, per=50.0, alpha=0.0, sigma=0.0, niter=5)
+
+
+def write_variants():
+    out_dir = os.path.join(HERE, "variants")
+    os.makedirs(out_dir, exist_ok=True)
+    for name, c in variant_cases():
+        img = np.ascontiguousarray(c["image"], np.int32)
+        lab0 = np.full(img.size, -1, np.int32)
+        lab0[c["seeds"]] = c["seed_labels"]
+        lab0 = lab0.reshape(img.shape)
+        G = grad_image(img)
+        extra = {}
+        if c["kind"] == "grad":
+            oift = O.ref_oift_gradweight(img, G, c["seeds"], lab0, c["adj_r"], c["alpha"], c["per"])
+            relax = O.ref_relax_gradweight(img, G, c["seeds"], oift, c["niter"], c["adj_r"], c["alpha"], c["per"])
+        else:
+            oift = O.ref_oift(img, c["seeds"], lab0, c["adj_r"], c["per"])
+            if c["kind"] == "rbf":
+                sigma = c["sigma"] if c["sigma"] > 0 else O.ref_estimate_sigma(img)
+                extra["sigma_used"] = np.float32(sigma)
+                relax = O.ref_relax_rbf(img, c["seeds"], oift, c["niter"], sigma, c["adj_r"], c["per"])
+            else:
+                form = c["form"]
+                lb = (oift > 0).astype(np.int32)
+                W = G if form == "W" else None
+                Wxyz = None
+                if form == "Wxyz":
+                    Wxyz = tuple(grad_image(np.roll(img, k, k)) for k in (2, 1, 0))
+                    extra.update(Wx=Wxyz[0], Wy=Wxyz[1], Wz=Wxyz[2])
+                seeds = None if form == "noseeds" else c["seeds"]
+                oift = lb
+                relax = O.ref_relax_binary(img, seeds, lb, c["niter"], c["adj_r"], c["per"], W=W, Wxyz=Wxyz)
+                extra["form"] = np.array(form)
+        np.savez_compressed(
+            os.path.join(out_dir, f"{name}.npz"), kind=np.array(c["kind"]), image=img.astype(np.int16), G=G,
+            seeds=c["seeds"].astype(np.int32), seed_labels=c["seed_labels"].astype(np.int32),
+            adj_r=np.float32(c["adj_r"]), per=np.float32(c["per"]), alpha=np.float32(c["alpha"]),
+            sigma=np.float32(c["sigma"]), niter=np.int32(c["niter"]), oift_label=oift.astype(np.int8),
+            relax_label=relax.astype(np.int8), **extra)
+        print(name, img.shape, "obj voxels", int((oift > 0).sum()), int((relax > 0).sum()))
+
+
 def main():
     if O.ref_lib() is None:
         raise SystemExit("oracle/_ref/libroift_ref.so is not built (make -C oracle ref)")
@@ -75,6 +151,7 @@ def main():
             niter=np.int32(c["niter"]), oift_label=oift.astype(np.int8 if oift.max() < 127 else np.int32),
             relax_label=relax.astype(np.int8))
         print(name, img.shape, "obj voxels", int((oift > 0).sum()), int((relax > 0).sum()))
+    write_variants()
 
 
 if __name__ == "__main__":

@@ -39,6 +39,11 @@ GFT_SIGNATURES = {
     "int)",
     "gft::ift::ORelax_1_Multi_PerClass(gft::AdjRel3::sAdjRel3*, gft::Scene32::sScene32*, float const*, int, int*, "
     "gft::Scene32::sScene32*, int)",
+    "gft::ift::ORelax_1(gft::AdjRel3::sAdjRel3*, gft::Scene32::sScene32*, float, int*, gft::Scene32::sScene32*, int)",
+    "gft::ift::ORelax_1(gft::Scene32::sScene32*, gft::AdjRel3::sAdjRel3*, gft::Scene32::sScene32*, float, int*, "
+    "gft::Scene32::sScene32*, int)",
+    "gft::ift::ORelax_1(gft::Scene32::sScene32*, gft::Scene32::sScene32*, gft::Scene32::sScene32*, "
+    "gft::Scene32::sScene32*, float, int*, gft::Scene32::sScene32*, int)",
 }
 
 
@@ -49,7 +54,7 @@ def _defined(path, dynamic=True):
 
 
 def test_gft_dropin_exports_reference_signatures():
-    """libroift_gft.so defines the four hot-path gft functions with the reference's mangled
+    """libroift_gft.so defines the hot-path gft functions (and the ORelax_1 forms) with the reference's mangled
     signatures, so a caller links it ahead of -lgft unchanged."""
     syms = _defined(os.path.join(ROOT, "roift_amd", "libroift_gft.so"))
     assert GFT_SIGNATURES <= syms
