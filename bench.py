@@ -4,9 +4,10 @@ One step = one pass of the hot path over one synthetic volume already resident
 in HBM: oriented weights + exact OIFT + 50 FP32 relaxation iterations + final
 labels (roift_pipeline_dev), preceded by restoring the seed-only label map.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--no-cpu] [--no-batch]
 
-Prints one JSON line.  The CPU baseline runs the reference's own
+Prints one JSON line.  Its "batched" object times the 64-volume batch (volumes of
+64-192 voxels a side) as 64 plain calls and as one batched call.  The CPU baseline runs the reference's own
 OIFT_Multi + ORelax_1_Multi (compiled from its sources into oracle/_ref) on one
 core over a bounded sample, or the oracle restatement when that build is absent.
 """
@@ -52,6 +53,56 @@ def cpu_baseline(size=128, runs=3):
                       f"{'reference gft sources -O3' if kind == 'reference' else 'oracle restatement -O3'}"}
 
 
+def batched_bench(ctx, reps=1):
+    """64 uneven volumes (dims log-uniform in [64, 192]), OIFT + 30 relaxation iterations each:
+    64 plain pipeline calls one after another vs one batched call (HIP-graph relaxation)."""
+    import torch
+    from roift_amd import phantom as P
+    from roift_amd.tune import autotune_oift
+    cases = P.batch_cases(count=64, seed=4, lo=64, hi=192, niter=30)
+    nvox = int(sum(c.image.size for c in cases))
+    imgs = [torch.from_numpy(c.image.astype(np.int16)).cuda() for c in cases]
+    seeds = [torch.from_numpy(c.seeds.astype(np.int32)).cuda() for c in cases]
+    lab0 = [torch.from_numpy(c.label_init()).cuda() for c in cases]
+    labs = [l.clone() for l in lab0]
+    per, niter = cases[0].per, cases[0].niter
+
+    def plain():
+        for img, s, l0, l in zip(imgs, seeds, lab0, labs):
+            l.copy_(l0)
+            ctx.pipeline(img, s, l, niter, 1.0, per)
+
+    def batched():
+        for l0, l in zip(lab0, labs):
+            l.copy_(l0)
+        ctx.batch_pipeline(imgs, seeds, labs, niter, per, use_graph=True, want_cost=False)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        best = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best
+
+    ctx.set_param("oift_blocks", 0)
+    ctx.set_param("flood_budget", 0)
+    t_plain = timed(plain)
+    tune = autotune_oift(ctx, "batch64x[64,192]^3", batched)
+    t_batch = timed(batched)
+    ctx.set_param("oift_blocks", 0)
+    ctx.set_param("flood_budget", 0)
+    return {"workload": "64 volumes, dims log-uniform in [64,192], oift_fmax + 30 relax iterations, 6-adj",
+            "voxels": nvox, "plain_mvox_s": round(nvox / t_plain / 1e6, 2),
+            "batched_mvox_s": round(nvox / t_batch / 1e6, 2), "speedup": round(t_plain / t_batch, 2),
+            "plain_ms": round(t_plain * 1e3, 1), "batched_ms": round(t_batch * 1e3, 1),
+            "oift_launch_shape": {"blocks": tune["blocks"], "budget": tune["budget"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +111,7 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe-copy", action="store_true", help="also run a 1 GiB device copy (profiler scaling)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-volume batched-vs-plain measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -152,6 +204,8 @@ def main():
         "other_rooflines": [roof("k_relax2_march", ALG["relax_iter"], relax_k),
                             roof("k_weights", ALG["weights"], weights_k)],
     }
+    if not args.no_batch:
+        res["batched"] = batched_bench(ctx)
     if args.probe_copy:
         # a plain copy of known size, for scaling the profiler's FETCH_SIZE / WRITE_SIZE
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev)

@@ -165,6 +165,20 @@ ROIFT_API int roift_pipeline_dev(roift_ctx *ctx, const roift_problem *pb, const 
                                  const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter,
                                  int32_t *d_cost, float *d_memb);
 
+/* Batched form: OIFT_Multi + ORelax_1_Multi (6-adjacency, polarity `per`) over nvol
+ * independent volumes stored back to back in d_image / d_label / d_cost (flat, volume v
+ * of h_dims[3v..3v+2] = xs, ys, zs voxels).  d_seeds holds each volume's seeds as
+ * volume-local indices, volume after volume, h_nseeds[v] of them.  Every volume's result
+ * equals roift_pipeline_dev on that volume alone (bit for bit).  One exact OIFT launch
+ * serves the whole batch; the relaxation runs one launch per iteration for all volumes
+ * (seed labels {0, L}, intensities < 32767; otherwise volume by volume), replayed as a
+ * HIP graph when use_graph != 0.  d_cost nullable.  No reference counterpart: the
+ * reference processes one volume per call (oiftrelax.cpp:379-401).                  */
+ROIFT_API int roift_batch_pipeline_dev(roift_ctx *ctx, int nvol, const int32_t *h_dims, float adj_radius, float per,
+                                       const void *d_image, int image_dtype, const int32_t *d_seeds,
+                                       const int32_t *h_nseeds, int32_t *d_label, int niter, int32_t *d_cost,
+                                       int use_graph);
+
 /* Host-pointer forms (copy in, run, copy out).  h_image is int32 as gft's Scene32. */
 ROIFT_API int roift_oift_fmax(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image,
                               const int32_t *h_seeds, int n_seeds, int32_t *h_label, int32_t *h_cost);

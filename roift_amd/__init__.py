@@ -28,7 +28,7 @@ EXPORTS = (
     "roift_weights_dev", "roift_oift_fmax_dev", "roift_oift_edges_dev", "roift_relax_dev",
     "roift_pipeline_dev", "roift_oift_fmax", "roift_relax", "oift_fmax_gpu_run",
     "roift_relax_binary_dev", "roift_oift_gradweight_dev", "roift_relax_gradweight_dev", "roift_relax_rbf_dev",
-    "roift_relax_binary",
+    "roift_relax_binary", "roift_batch_pipeline_dev",
 )
 
 
@@ -80,6 +80,7 @@ def lib():
         L.roift_relax_gradweight_dev.argtypes = [vp, pp, vp, ip, vp, fl, vp, ip, vp, ip, vp]
         L.roift_relax_rbf_dev.argtypes = [vp, pp, vp, ip, fl, vp, ip, vp, ip, vp, fp]
         L.roift_relax_binary.argtypes = [vp, pp, vp, vp, vp, vp, vp, vp, ip, vp, ip, vp]
+        L.roift_batch_pipeline_dev.argtypes = [vp, ip, i32p, fl, fl, vp, ip, vp, i32p, vp, ip, vp, ip]
         _lib = L
     return _lib
 
@@ -281,6 +282,32 @@ class Context:
                                                 label.data_ptr(), int(niter),
                                                 memb.data_ptr() if memb is not None else None, C.byref(used)))
         return memb, used.value
+
+    def batch_pipeline(self, images, seeds, labels, niter, per=0.0, use_graph=True, want_cost=True):
+        """OIFT_Multi + ORelax_1_Multi over a list of volumes in one batched call.
+
+        images: list of device int16/int32 tensors (zs, ys, xs); seeds: list of int32 device
+        tensors (volume-local indices); labels: list of int32 device tensors (in/out).
+        Returns the list of cost maps (or None)."""
+        import torch
+        dev = labels[0].device
+        dims = np.array([[t.shape[2], t.shape[1], t.shape[0]] for t in images], np.int32).ravel()
+        nseeds = np.array([int(s.numel()) for s in seeds], np.int32)
+        img = torch.cat([t.reshape(-1) for t in images]).contiguous()
+        S = torch.cat([s.to(device=dev, dtype=torch.int32).reshape(-1) for s in seeds]).contiguous()
+        lab = torch.cat([t.reshape(-1) for t in labels]).contiguous()
+        cost = torch.empty_like(lab) if want_cost else None
+        self._check(self._L.roift_batch_pipeline_dev(
+            self._h, len(images), dims.ctypes.data_as(C.POINTER(C.c_int32)), 1.0, float(per), img.data_ptr(),
+            _dtype_code(images[0]), S.data_ptr(), nseeds.ctypes.data_as(C.POINTER(C.c_int32)), lab.data_ptr(),
+            int(niter), cost.data_ptr() if cost is not None else None, int(bool(use_graph))))
+        out, o = [], 0
+        for t in labels:
+            k = t.numel()
+            t.copy_(lab[o:o + k].view(t.shape))
+            out.append(None if cost is None else cost[o:o + k].view(t.shape))
+            o += k
+        return out
 
     def pipeline(self, image, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None, cost=None, memb=None):
         """weights + OIFT + relax + labels on device tensors (label in/out)."""

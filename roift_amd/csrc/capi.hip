@@ -165,6 +165,7 @@ ROIFT_API void roift_destroy(roift_ctx *c)
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->h_ctr) (void)hipHostFree(c->h_ctr);
+    if (c->relax_graph) (void)hipGraphExecDestroy(c->relax_graph);
     if (c->owns_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -381,6 +382,17 @@ ROIFT_API int roift_relax_rbf_dev(roift_ctx *ctx, const roift_problem *pb, const
         ev_record(*ctx, 3);
         ctx->timings[2] = ev_ms(*ctx, 2, 3);
         return ROIFT_OK;
+    });
+}
+
+ROIFT_API int roift_batch_pipeline_dev(roift_ctx *ctx, int nvol, const int32_t *h_dims, float adj_radius, float per,
+                                       const void *d_image, int image_dtype, const int32_t *d_seeds,
+                                       const int32_t *h_nseeds, int32_t *d_label, int niter, int32_t *d_cost,
+                                       int use_graph)
+{
+    return guarded(ctx, [&]() -> int {
+        return run_batch(*ctx, nvol, h_dims, adj_radius, per, d_image, image_dtype, d_seeds, h_nseeds, d_label, niter,
+                         d_cost, use_graph != 0);
     });
 }
 

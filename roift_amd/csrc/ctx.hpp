@@ -19,6 +19,8 @@ struct roift_ctx {
     int tune_budget = 0;            // OIFT LDS batches per workgroup per flood round (0: default)
     int tune_relax_depth = 0;       // relaxation temporal-block depth (0: default)
     hipEvent_t ev[16] = {};
+    hipGraphExec_t relax_graph = nullptr;   // batched relaxation iterations, replayed while the batch is unchanged
+    std::string relax_graph_key;
     // device-event times of the last call's main kernels, ms: weight builder, OIFT
     // persistent kernel, all relaxation iterations; [3] = relaxation launches
     double ktimes[4] = {0, 0, 0, 0};
@@ -118,6 +120,27 @@ struct RelaxModel {
 int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, const RelaxSpec &rs,
               const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter, float *d_memb, int32_t *nclasses_out,
               const WeightInfo &wi, const RelaxModel &rm = RelaxModel());
+
+// One volume of a batch stored back to back: voxel offset, size and image extrema.
+struct BatchVolume {
+    long long off;
+    int xs, ys, zs;
+    int32_t maxI, minI;
+};
+
+// Two-class 6-adjacency ORelax_1_Multi over a batch in one launch per iteration (labels
+// {0, lab_obj}); d_seeds are flat (batch-global) indices.  use_graph replays the
+// iteration launches as a HIP graph.
+int run_relax_batch(roift_ctx &c, const std::vector<BatchVolume> &vols, const void *d_image, int image_dtype,
+                    float per, int lab_obj, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter,
+                    bool use_graph);
+
+// OIFT_Multi + ORelax_1_Multi (6-adjacency, one polarity) over nvol independent volumes
+// stored back to back: h_dims = xs, ys, zs per volume; d_seeds = each volume's seeds as
+// volume-local indices, volume after volume (h_nseeds[v] of them); d_label / d_cost flat.
+int run_batch(roift_ctx &c, int nvol, const int32_t *h_dims, float adj_r, float per, const void *d_image,
+              int image_dtype, const int32_t *d_seeds, const int32_t *h_nseeds, int32_t *d_label, int niter,
+              int32_t *d_cost, bool use_graph);
 
 // ORelax_1 (binary fallback): single channel f = label/Imax, hard polarity.  Weight source:
 // the image (W == Wx == nullptr), a per-voxel weight W, or per-axis Wx/Wy/Wz (6-adj only).

@@ -18,6 +18,7 @@
 // Bound: HBM.  Algorithmic bytes per voxel and iteration: image read
 // (2 B int16) + membership read (4 B/channel) + membership write (4 B/channel).
 #include <algorithm>
+#include <cstdio>
 
 #include "ctx.hpp"
 
@@ -255,15 +256,53 @@ __device__ inline float relax_w8(float d, float fq, const RelaxArgs &a)
     return w * w;
 }
 
+// Batched form: one launch relaxes a batch of volumes stored one after another
+// (flat, unpadded).  Block b takes tile tiles[b] of volume tiles[b].vol, whose
+// offset, size and Wmax come from vols[].
+struct MarchTile {
+    int vol, x0, y0, z0;
+};
+struct MarchVol {
+    long long off;
+    int xs, ys, zs;
+    float wmax;
+};
+
+template <bool BATCH>
 __global__ __launch_bounds__(MX *MY) void k_relax2_march(const int16_t *__restrict__ img, RelaxArgs a,
+                                                          const MarchTile *__restrict__ tiles,
+                                                          const MarchVol *__restrict__ vols,
                                                           const float *__restrict__ fin, float *__restrict__ fout,
                                                           int32_t *__restrict__ label_out)
 {
-    const int xs = a.d.xs, ys = a.d.ys, zs = a.d.zs;
-    const int x = blockIdx.x * MX + threadIdx.x, y = blockIdx.y * MY + threadIdx.y;
-    const int z0 = blockIdx.z * MZ, z1 = min(zs, z0 + MZ);
+    int xs, ys, zs, x, y, z0;
+    long long base = 0;
+    if constexpr (BATCH) {
+        const MarchTile t = tiles[blockIdx.x];
+        const MarchVol mv = vols[t.vol];
+        xs = mv.xs;
+        ys = mv.ys;
+        zs = mv.zs;
+        base = mv.off;
+        a.Wmax = mv.wmax;
+        x = t.x0 + (int)threadIdx.x;
+        y = t.y0 + (int)threadIdx.y;
+        z0 = t.z0;
+    } else {
+        xs = a.d.xs;
+        ys = a.d.ys;
+        zs = a.d.zs;
+        x = blockIdx.x * MX + threadIdx.x;
+        y = blockIdx.y * MY + threadIdx.y;
+        z0 = blockIdx.z * MZ;
+    }
+    const int z1 = min(zs, z0 + MZ);
     if (x >= xs || y >= ys || z0 >= zs) return;   // no block-level synchronisation below
     const int xy = xs * ys;
+    img += base;
+    fin += base;
+    fout += base;
+    if (label_out) label_out += base;
     int p = x + y * xs + z0 * xy;
     const int lab_bg = a.cls_label[a.bg], lab_ob = a.cls_label[1 - a.bg];
     bool hm = z0 > 0;
@@ -482,6 +521,26 @@ void launch_bin(roift_ctx &c, int bm, const void *img, const BinArgs &a, const f
     ROIFT_HIP_CHECK(hipGetLastError());
 }
 
+// ---------------------------------------------------------------- batched two-class relaxation
+__global__ void k_batch_finit(const int32_t *__restrict__ label, long long n, int lab_obj, float *__restrict__ f)
+{
+    for (long long p = gtid(); p < n; p += gstride()) f[p] = (label[p] == lab_obj) ? 0.0f : 1.0f;
+}
+
+template <typename IT>
+__global__ void k_batch_pack(const IT *__restrict__ img, long long n, int16_t *__restrict__ out)
+{
+    for (long long p = gtid(); p < n; p += gstride()) out[p] = (int16_t)((int)img[p] & 0x7FFF);
+}
+
+__global__ void k_batch_seedbits(const int32_t *__restrict__ S, int ns, int16_t *__restrict__ packed)
+{
+    for (long long i = gtid(); i < ns; i += gstride()) {
+        const int v = S[i];
+        packed[v] = (int16_t)((int)packed[v] | 0x8000);
+    }
+}
+
 }  // namespace
 
 int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, const RelaxSpec &rs,
@@ -604,7 +663,7 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
     for (int it = 0; it < niter; it++) {
         int32_t *lo = (it == niter - 1) ? d_label : nullptr;
         if (march) {
-            hipLaunchKernelGGL(k_relax2_march, mgrid, dim3(MX, MY), 0, st, pimg, a, fin, fout, lo);
+            hipLaunchKernelGGL(k_relax2_march<false>, mgrid, dim3(MX, MY), 0, st, pimg, a, nullptr, nullptr, fin, fout, lo);
             ROIFT_HIP_CHECK(hipGetLastError());
         } else if (nadj == 7) {
             if (i16) launch_iter<7, int16_t>(c, d_image, a, rm.model, fin, fout, seedcls, lo);
@@ -713,6 +772,99 @@ int run_relax_binary(roift_ctx &c, const Dims &d, int nadj, const void *d_image,
     ROIFT_HIP_CHECK(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));
     ROIFT_HIP_CHECK(hipStreamSynchronize(st));
     ROIFT_REQUIRE(hinfo[1] == 0, ROIFT_EINVAL, "seed index out of range");
+    c.ktimes[2] = ev_ms(c, 12, 13);
+    c.ktimes[3] = niter;
+    return 0;
+}
+
+}  // namespace roift
+
+namespace roift {
+
+int run_relax_batch(roift_ctx &c, const std::vector<BatchVolume> &vols, const void *d_image, int image_dtype,
+                    float per, int lab_obj, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter,
+                    bool use_graph)
+{
+    ROIFT_REQUIRE(!vols.empty() && niter >= 0, ROIFT_EINVAL, "empty batch or negative iteration count");
+    hipStream_t st = c.stream;
+    long long n = 0;
+    std::vector<MarchVol> mv;
+    std::vector<MarchTile> tiles;
+    for (size_t v = 0; v < vols.size(); v++) {
+        const BatchVolume &b = vols[v];
+        ROIFT_REQUIRE(b.off == n, ROIFT_EINVAL, "batch volumes must be stored back to back");
+        ROIFT_REQUIRE(b.minI >= 0 && b.maxI < 0x7FFF, ROIFT_EUNSUPPORTED, "batched relaxation needs 0 <= I < 32767");
+        // float Wmax = (float)Imax; Wmax *= (1.0 + |per|/100.0)   (gft_ift.cpp:8733-8734), per volume
+        mv.push_back(MarchVol{b.off, b.xs, b.ys, b.zs,
+                              (float)((double)(float)b.maxI * (1.0 + (double)std::fabs(per) / 100.0))});
+        for (int z = 0; z < b.zs; z += MZ)
+            for (int y = 0; y < b.ys; y += MY)
+                for (int x = 0; x < b.xs; x += MX) tiles.push_back(MarchTile{(int)v, x, y, z});
+        n += (long long)b.xs * b.ys * b.zs;
+    }
+    ROIFT_REQUIRE(n < (1ll << 31), ROIFT_EINVAL, "batch larger than 2^31-1 voxels");
+    MarchVol *dvol = buf<MarchVol>(c, "batch.vols", mv.size());
+    MarchTile *dtile = buf<MarchTile>(c, "batch.tiles", tiles.size());
+    ROIFT_HIP_CHECK(hipMemcpyAsync(dvol, mv.data(), mv.size() * sizeof(MarchVol), hipMemcpyHostToDevice, st));
+    ROIFT_HIP_CHECK(hipMemcpyAsync(dtile, tiles.data(), tiles.size() * sizeof(MarchTile), hipMemcpyHostToDevice, st));
+    RelaxArgs a{};
+    a.d = Dims{0, 0, 0, n};
+    a.per = per;
+    a.nc = 2;
+    a.bg = 0;   // classes {0, lab_obj} in increasing label order
+    a.cls_label[0] = 0;
+    a.cls_label[1] = lab_obj;
+    a.tA = (float)(2.0 * (double)per / 100.0);
+    a.tB = (float)(-(double)per / 100.0);
+    float *f0 = buf<float>(c, "relax.f0", n);
+    float *f1 = buf<float>(c, "relax.f1", n);
+    int16_t *pimg = buf<int16_t>(c, "relax.pimg", n);
+    const unsigned G = grid_for(n, 256, 8192);
+    hipLaunchKernelGGL(k_batch_finit, dim3(G), dim3(256), 0, st, d_label, n, lab_obj, f0);
+    if (image_dtype == ROIFT_I16)
+        hipLaunchKernelGGL(k_batch_pack<int16_t>, dim3(G), dim3(256), 0, st, (const int16_t *)d_image, n, pimg);
+    else
+        hipLaunchKernelGGL(k_batch_pack<int32_t>, dim3(G), dim3(256), 0, st, (const int32_t *)d_image, n, pimg);
+    if (n_seeds > 0)
+        hipLaunchKernelGGL(k_batch_seedbits, dim3(grid_for(n_seeds)), dim3(256), 0, st, d_seeds, n_seeds, pimg);
+    ROIFT_HIP_CHECK(hipGetLastError());
+    const unsigned nt = (unsigned)tiles.size();
+    auto launch_all = [&]() {
+        float *fin = f0, *fout = f1;
+        for (int it = 0; it < niter; it++) {
+            int32_t *lo = (it == niter - 1) ? d_label : nullptr;
+            hipLaunchKernelGGL(k_relax2_march<true>, dim3(nt), dim3(MX, MY), 0, st, pimg, a, dtile, dvol, fin, fout,
+                               lo);
+            std::swap(fin, fout);
+        }
+    };
+    ev_record(c, 12);
+    bool graphed = false;
+    if (use_graph && niter > 0 && st != nullptr) {
+        // the iteration launches as one HIP graph, re-instantiated when the batch changes
+        char key[256];
+        std::snprintf(key, sizeof(key), "%p %p %p %p %p %p %u %d %lld %a", (void *)pimg, (void *)dtile,
+                      (void *)dvol, (void *)f0, (void *)f1, (void *)d_label, nt, niter, n, (double)per);
+        if (!c.relax_graph || c.relax_graph_key != key) {
+            if (c.relax_graph) {
+                ROIFT_HIP_CHECK(hipGraphExecDestroy(c.relax_graph));
+                c.relax_graph = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            ROIFT_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            launch_all();
+            ROIFT_HIP_CHECK(hipStreamEndCapture(st, &g));
+            ROIFT_HIP_CHECK(hipGraphInstantiate(&c.relax_graph, g, nullptr, nullptr, 0));
+            ROIFT_HIP_CHECK(hipGraphDestroy(g));
+            c.relax_graph_key = key;
+        }
+        ROIFT_HIP_CHECK(hipGraphLaunch(c.relax_graph, st));
+        graphed = true;
+    }
+    if (!graphed) launch_all();
+    ROIFT_HIP_CHECK(hipGetLastError());
+    ev_record(c, 13);
+    ROIFT_HIP_CHECK(hipStreamSynchronize(st));
     c.ktimes[2] = ev_ms(c, 12, 13);
     c.ktimes[3] = niter;
     return 0;

@@ -1,0 +1,100 @@
+"""GPU: the batched pipeline (roift_batch_pipeline_dev) gives every volume exactly the
+result of the plain per-volume pipeline, including on HIP-graph replays interleaved with
+other eager work, and the OIFT part equals the oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+from roift_amd import phantom as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _upload(cases):
+    import torch
+    imgs = [torch.from_numpy(c.image.astype(np.int16)).cuda() for c in cases]
+    seeds = [torch.from_numpy(c.seeds.astype(np.int32)).cuda() for c in cases]
+    labs = [torch.from_numpy(c.label_init()).cuda() for c in cases]
+    return imgs, seeds, labs
+
+
+def _plain(ctx, cases, niter):
+    import torch
+    imgs, seeds, labs = _upload(cases)
+    out = []
+    for img, s, lab in zip(imgs, seeds, labs):
+        cost = torch.empty_like(lab)
+        ctx.pipeline(img, s, lab, niter, 1.0, cases[0].per, cost=cost)
+        out.append((lab.cpu().numpy(), cost.cpu().numpy()))
+    return out
+
+
+def _batched(ctx, cases, niter, use_graph):
+    imgs, seeds, labs = _upload(cases)
+    costs = ctx.batch_pipeline(imgs, seeds, labs, niter, cases[0].per, use_graph=use_graph)
+    return [(l.cpu().numpy(), c.cpu().numpy()) for l, c in zip(labs, costs)]
+
+
+def _same(a, b):
+    for i, ((la, ca), (lb, cb)) in enumerate(zip(a, b)):
+        assert np.array_equal(la, lb), f"labels differ in volume {i}"
+        assert np.array_equal(ca, cb), f"costs differ in volume {i}"
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_equals_plain_small(ctx, use_graph):
+    cases = P.batch_cases(count=7, seed=11, lo=10, hi=36, niter=6)
+    print("volumes", [c.image.shape for c in cases], flush=True)
+    plain = _plain(ctx, cases, 6)
+    _same(_batched(ctx, cases, 6, use_graph), plain)
+
+
+def test_batch_oift_equals_oracle(ctx):
+    cases = P.batch_cases(count=4, seed=12, lo=10, hi=28, niter=0)
+    got = _batched(ctx, cases, 0, False)
+    for c, (lab, cost) in zip(cases, got):
+        o = O.oift_image(c.image, c.seeds, c.label_init(), 1.0, c.per)
+        assert np.array_equal(lab, o["label"]) and np.array_equal(cost, o["cost"])
+
+
+def test_batch_graph_replays_with_eager_work_between(ctx):
+    """Replay the cached graph several times with other eager launches and syncs in between;
+    every replay must equal the eager per-volume result."""
+    import torch
+    cases = P.batch_cases(count=5, seed=13, lo=12, hi=30, niter=5)
+    plain = _plain(ctx, cases, 5)
+    for rep in range(3):
+        _same(_batched(ctx, cases, 5, True), plain)
+        # unrelated eager work on the same context and stream
+        other = P.small_case((14, 15, 16), 50 + rep)
+        lab = torch.from_numpy(other.label_init()).cuda()
+        ctx.pipeline(torch.from_numpy(other.image.astype(np.int16)).cuda(),
+                     torch.from_numpy(other.seeds.astype(np.int32)).cuda(), lab, 3, 1.0, other.per)
+        torch.cuda.synchronize()
+
+
+def test_batch_three_labels_falls_back_per_volume(ctx):
+    """Seed labels {0, 1, 2} take the per-volume relaxation inside the batch call."""
+    cases = P.batch_cases(count=3, seed=14, lo=12, hi=26, niter=4)
+    for c in cases:   # relabel the first internal seed as label 2
+        c.seed_labels = c.seed_labels.copy()
+        c.seed_labels[0] = 2
+    plain = _plain(ctx, cases, 4)
+    _same(_batched(ctx, cases, 4, True), plain)
+
+
+def test_batch_rejects_bad_seed(ctx):
+    import roift_amd as R
+    cases = P.batch_cases(count=2, seed=15, lo=10, hi=14, niter=2)
+    imgs, seeds, labs = _upload(cases)
+    seeds[1][0] = cases[1].image.size + 5   # outside its own volume
+    with pytest.raises(R.RoiftError):
+        ctx.batch_pipeline(imgs, seeds, labs, 2, cases[0].per, use_graph=False)
+
+
+def test_batch_headline_config_64_volumes(ctx):
+    """64 uneven volumes, dims log-uniform in [64, 192], 30 relaxation iterations: batch == plain."""
+    cases = P.batch_cases(count=64, seed=4, lo=64, hi=192, niter=30)
+    print("total voxels", sum(c.image.size for c in cases), flush=True)
+    plain = _plain(ctx, cases, 30)
+    _same(_batched(ctx, cases, 30, True), plain)
