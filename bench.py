@@ -4,10 +4,12 @@ One step = one pass of the hot path over one synthetic volume already resident
 in HBM: oriented weights + exact OIFT + 50 FP32 relaxation iterations + final
 labels (roift_pipeline_dev), preceded by restoring the seed-only label map.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--no-cpu] [--no-batch]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--no-cpu] [--no-batch] [--no-adjoint]
 
 Prints one JSON line.  Its "batched" object times the 64-volume batch (volumes of
-64-192 voxels a side) as 64 plain calls and as one batched call.  The CPU baseline runs the reference's own
+64-192 voxels a side) as 64 plain calls and as one batched call; its "adjoint" object
+times the differentiable relaxation's forward + checkpointed adjoint on 16 x 128^3.  The
+CPU baseline runs the reference's own
 OIFT_Multi + ORelax_1_Multi (compiled from its sources into oracle/_ref) on one
 core over a bounded sample, or the oracle restatement when that build is absent.
 """
@@ -103,6 +105,44 @@ def batched_bench(ctx, reps=1):
             "oift_launch_shape": {"blocks": tune["blocks"], "budget": tune["budget"]}}
 
 
+def adjoint_bench(ctx, count=16, size=128, niter=50, ckpt=10):
+    """16 volumes of 128^3: exact OIFT, then the differentiable relaxation (50 iterations)
+    forward + checkpointed adjoint (every 10) against the Gaussian(sigma=2)-blurred true mask;
+    learnable per-voxel scale 1.0, RBF sigma = the image's global std, alpha 0.5."""
+    import torch
+    import scipy.ndimage as nd
+    from roift_amd import phantom as P
+    vols = []
+    for i in range(count):
+        c = P.headline_case(size, seed=100 + i, niter=niter)
+        img = torch.from_numpy(c.image.astype(np.int16)).cuda()
+        seeds = torch.from_numpy(c.seeds.astype(np.int32)).cuda()
+        lab = torch.from_numpy(c.label_init()).cuda()
+        ctx.oift_fmax(img, seeds, lab, 1.0, c.per, want_cost=False)
+        target = torch.from_numpy(nd.gaussian_filter(c.truth.astype(np.float32), 2.0)).cuda()
+        scale = torch.ones(c.image.shape, dtype=torch.float32, device="cuda")
+        sigma = float(np.float32(np.sqrt(c.image.astype(np.float64).var())))
+        vols.append((img, seeds, lab, scale, sigma, target))
+
+    def run():
+        out = []
+        for img, seeds, lab, scale, sigma, target in vols:
+            out.append(ctx.relax_adjoint(img, seeds, lab, 1, niter, scale, sigma, 0.5, target, ckpt_every=ckpt)[0])
+        return out
+
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = run()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    nvox = count * size ** 3
+    return {"workload": f"{count} x {size}^3, differentiable relax {niter} iterations fwd + adjoint, "
+                        f"checkpoint every {ckpt}, BCE vs blurred truth",
+            "voxels": nvox, "fwd_bwd_mvox_s": round(nvox / dt / 1e6, 2), "ms": round(dt * 1e3, 1),
+            "mean_loss": round(float(np.mean(losses)), 6)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +152,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe-copy", action="store_true", help="also run a 1 GiB device copy (profiler scaling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-volume batched-vs-plain measurement")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 16 x 128^3 relaxation-adjoint measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -206,6 +247,8 @@ def main():
     }
     if not args.no_batch:
         res["batched"] = batched_bench(ctx)
+    if not args.no_adjoint:
+        res["adjoint"] = adjoint_bench(ctx)
     if args.probe_copy:
         # a plain copy of known size, for scaling the profiler's FETCH_SIZE / WRITE_SIZE
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev)

@@ -179,6 +179,22 @@ ROIFT_API int roift_batch_pipeline_dev(roift_ctx *ctx, int nvol, const int32_t *
                                        const int32_t *h_nseeds, int32_t *d_label, int niter, int32_t *d_cost,
                                        int use_graph);
 
+/* Differentiable two-class relaxation and its adjoint.  No reference counterpart: the
+ * learnable weight joins ORelax_1_Multi_GaussRBF's Gaussian (exp_gaussian_rbf_relax.cpp:
+ * 343-345) with ORelax_1_Multi's soft polarity (gft_ift.cpp:8800-8812) and a per-voxel
+ * scale:  w_pq = scale_q exp(-(I_p-I_q)^2/sigma^2) (1 + alpha sgn(I_p-I_q)(2 f_q - 1)) / Dpq,
+ * f = background membership (0 on label == lab_obj, 1 elsewhere at the start), seeds
+ * fixed, niter Jacobi iterations.  Loss = binary cross-entropy of o = 1 - f against
+ * d_target (N floats in [0,1]), o clamped to [eps, 1-eps].  Returns the loss and
+ * dLoss/dscale (d_grad_scale, N floats), dLoss/dsigma, dLoss/dalpha by reverse mode,
+ * storing one state every ckpt_every iterations and recomputing each segment.
+ * d_memb (nullable) receives the final f.                                          */
+ROIFT_API int roift_relax_adjoint_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                      const int32_t *d_seeds, int n_seeds, const int32_t *d_label, int lab_obj,
+                                      int niter, int ckpt_every, const float *d_scale, float sigma, float alpha,
+                                      const float *d_target, float eps, double *h_loss, float *d_grad_scale,
+                                      double *h_grad_sigma, double *h_grad_alpha, float *d_memb);
+
 /* Host-pointer forms (copy in, run, copy out).  h_image is int32 as gft's Scene32. */
 ROIFT_API int roift_oift_fmax(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image,
                               const int32_t *h_seeds, int n_seeds, int32_t *h_label, int32_t *h_cost);

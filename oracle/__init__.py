@@ -65,6 +65,9 @@ def lib():
         L.orc_gradweight_edges.argtypes = [i32p, i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                                            u32p, u32p]
         L.orc_estimate_sigma.argtypes = [i32p, C.c_int64]
+        L.orc_relax_diff_f64.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_float, i32p, i32p, C.c_int, C.c_int,
+                                         f64p, C.c_double, C.c_double, f64p, C.c_double, f64p, f64p, f64p, f64p,
+                                         f64p]
         L.orc_estimate_sigma.restype = C.c_float
         _LIB = L
     return _LIB
@@ -278,6 +281,27 @@ def oift_gradweight(I, G, seeds, label, adj_r=1.0, alpha=0.5, per=0.0):
 def estimate_sigma(I):
     I = _i32(I)
     return float(lib().orc_estimate_sigma(_p(I, C.c_int32), I.size))
+
+
+def relax_diff(I, seeds, label, lab_obj, niter, scale, sigma, alpha, target, adj_r=1.0, eps=1e-6):
+    """FP64 differentiable two-class relaxation (orc_relax_diff_f64): returns dict(loss,
+    g_scale, g_sigma, g_alpha, f)."""
+    I = _i32(I)
+    xs, ys, zs = _dims(I)
+    lab = _i32(label)
+    S = _gft_S(seeds)
+    sc = np.ascontiguousarray(scale, np.float64)
+    tg = np.ascontiguousarray(target, np.float64)
+    gsc = np.empty(I.shape, np.float64)
+    f = np.empty(I.shape, np.float64)
+    loss, gs, ga = np.zeros(1), np.zeros(1), np.zeros(1)
+    rc = lib().orc_relax_diff_f64(_p(I, C.c_int32), xs, ys, zs, adj_r, _p(S, C.c_int32), _p(lab, C.c_int32),
+                                  int(lab_obj), int(niter), _p(sc, C.c_double), float(sigma), float(alpha),
+                                  _p(tg, C.c_double), float(eps), _p(loss, C.c_double), _p(gsc, C.c_double),
+                                  _p(gs, C.c_double), _p(ga, C.c_double), _p(f, C.c_double))
+    if rc != 0:
+        raise ValueError(f"orc_relax_diff_f64 failed rc={rc}")
+    return dict(loss=float(loss[0]), g_scale=gsc, g_sigma=float(gs[0]), g_alpha=float(ga[0]), f=f)
 
 
 # ---- the compiled reference (only where oracle/_ref was built) -------------------

@@ -811,3 +811,118 @@ int orc_relax_binary_f32(const int32_t *I, int xs, int ys, int zs, float adj_r, 
 {
     return orc_relax_binary_w_f32(I, NULL, NULL, NULL, NULL, xs, ys, zs, adj_r, per, S, label, ntimes, 0, memb);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Differentiable two-class relaxation (no reference counterpart: the       */
+/* learnable weight is defined by this project; this FP64 restatement is    */
+/* its oracle).  f = background membership, init 0 on label == lab_obj and  */
+/* 1 elsewhere; seeds keep their initial value.  For a non-seed voxel p:    */
+/*   d = I_p - I_q,  g = exp(-d^2/sigma^2),                                  */
+/*   pi = 1 + alpha*sgn(d)*(2 f_q - 1)        (soft polarity, as ORelax_1)  */
+/*   w_pq = scale_q * g * pi / Dpq,                                          */
+/*   f'_p = sum_q w_pq f_q / sum_q w_pq   (f'_p = f_p when the sum is 0).    */
+/* Loss: binary cross-entropy of the object membership o = 1 - f against   */
+/* target t, o clamped to [eps, 1-eps] (zero derivative where clamped):     */
+/*   L = -(1/N) sum_p [t log o + (1-t) log(1-o)].                            */
+/* Outputs L, dL/dscale (N), dL/dsigma, dL/dalpha by reverse mode over the  */
+/* stored states, and the final f.                                          */
+/* ------------------------------------------------------------------------ */
+int orc_relax_diff_f64(const int32_t *I, int xs, int ys, int zs, float adj_r, const int32_t *S,
+                       const int32_t *label, int lab_obj, int niter, const double *scale, double sigma,
+                       double alpha, const double *target, double eps, double *loss, double *g_scale,
+                       double *g_sigma, double *g_alpha, double *f_out)
+{
+    orc_adj A;
+    if (adj_make(adj_r, &A) != ORC_OK) return ORC_EINVAL;
+    const int64_t n64 = (int64_t)xs * ys * zs;
+    if (n64 > INT_MAX || n64 <= 0 || niter < 0) return ORC_EINVAL;
+    const int n = (int)n64, xy = xs * ys;
+    double Dpq[125];
+    for (int i = 1; i < A.n; i++)
+        Dpq[i] = sqrt((double)(A.dx[i] * A.dx[i] + A.dy[i] * A.dy[i] + A.dz[i] * A.dz[i]));
+    double *F = (double *)malloc(sizeof(double) * (size_t)n * (size_t)(niter + 1));
+    double *lam = (double *)calloc((size_t)n, sizeof(double)), *lam2 = (double *)calloc((size_t)n, sizeof(double));
+    char *seed = (char *)calloc((size_t)n, 1);
+    if (!F || !lam || !lam2 || !seed) { free(F); free(lam); free(lam2); free(seed); return ORC_ENOMEM; }
+    if (S) for (int i = 1; i <= S[0]; i++) if (S[i] >= 0 && S[i] < n) seed[S[i]] = 1;
+    for (int p = 0; p < n; p++) F[p] = (label[p] == lab_obj) ? 0.0 : 1.0;
+    const double s2 = sigma * sigma;
+#define DIFF_W(p, q, i, fq, g_, sg_, pi_, w_)                                   \
+    double d_ = (double)(I[p] - I[q]);                                        \
+    double g_ = exp(-(d_ * d_) / s2);                                         \
+    double sg_ = (d_ > 0) ? 1.0 : ((d_ < 0) ? -1.0 : 0.0);                    \
+    double pi_ = 1.0 + alpha * sg_ * (2.0 * (fq) - 1.0);                      \
+    double w_ = scale[q] * g_ * pi_ / Dpq[i];
+    for (int k = 0; k < niter; k++) {
+        const double *f = F + (size_t)k * n;
+        double *fn = F + (size_t)(k + 1) * n;
+        for (int p = 0; p < n; p++) {
+            if (seed[p]) { fn[p] = f[p]; continue; }
+            int px = (p % xy) % xs, py = (p % xy) / xs, pz = p / xy;
+            double a = 0.0, b = 0.0;
+            for (int i = 1; i < A.n; i++) {
+                int qx = px + A.dx[i], qy = py + A.dy[i], qz = pz + A.dz[i];
+                if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) continue;
+                int q = qx + qy * xs + qz * xy;
+                DIFF_W(p, q, i, f[q], g, sg, pi, w)
+                (void)g; (void)sg; (void)pi;
+                a += w * f[q];
+                b += w;
+            }
+            fn[p] = (b != 0.0) ? a / b : f[p];
+        }
+    }
+    const double *ff = F + (size_t)niter * n;
+    double L = 0.0;
+    for (int p = 0; p < n; p++) {
+        double o = 1.0 - ff[p];
+        int clamped = 0;
+        if (o < eps) { o = eps; clamped = 1; }
+        if (o > 1.0 - eps) { o = 1.0 - eps; clamped = 1; }
+        L -= target[p] * log(o) + (1.0 - target[p]) * log(1.0 - o);
+        /* dL/df = -dL/do, dL/do = -(1/N)(t/o - (1-t)/(1-o)) */
+        lam[p] = clamped ? 0.0 : (target[p] / o - (1.0 - target[p]) / (1.0 - o)) / (double)n;
+    }
+    L /= (double)n;
+    double gs = 0.0, ga = 0.0;
+    for (int p = 0; p < n; p++) g_scale[p] = 0.0;
+    for (int k = niter - 1; k >= 0; k--) {
+        const double *f = F + (size_t)k * n, *fn = F + (size_t)(k + 1) * n;
+        for (int p = 0; p < n; p++) lam2[p] = 0.0;
+        for (int p = 0; p < n; p++) {
+            if (seed[p] || lam[p] == 0.0) continue;
+            int px = (p % xy) % xs, py = (p % xy) / xs, pz = p / xy;
+            double b = 0.0;
+            for (int i = 1; i < A.n; i++) {
+                int qx = px + A.dx[i], qy = py + A.dy[i], qz = pz + A.dz[i];
+                if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) continue;
+                int q = qx + qy * xs + qz * xy;
+                DIFF_W(p, q, i, f[q], g, sg, pi, w)
+                (void)g; (void)sg; (void)pi;
+                b += w;
+            }
+            if (b == 0.0) { lam2[p] += lam[p]; continue; }
+            const double c = lam[p] / b;
+            for (int i = 1; i < A.n; i++) {
+                int qx = px + A.dx[i], qy = py + A.dy[i], qz = pz + A.dz[i];
+                if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) continue;
+                int q = qx + qy * xs + qz * xy;
+                DIFF_W(p, q, i, f[q], g, sg, pi, w)
+                const double common = f[q] - fn[p];
+                const double dwdf = scale[q] * g * alpha * sg * 2.0 / Dpq[i];
+                lam2[q] += c * (w + common * dwdf);
+                g_scale[q] += c * common * g * pi / Dpq[i];
+                gs += c * common * w * 2.0 * d_ * d_ / (s2 * sigma);
+                ga += c * common * scale[q] * g * sg * (2.0 * f[q] - 1.0) / Dpq[i];
+            }
+        }
+        double *t = lam; lam = lam2; lam2 = t;
+    }
+#undef DIFF_W
+    *loss = L;
+    *g_sigma = gs;
+    *g_alpha = ga;
+    if (f_out) memcpy(f_out, ff, sizeof(double) * (size_t)n);
+    free(F); free(lam); free(lam2); free(seed);
+    return ORC_OK;
+}

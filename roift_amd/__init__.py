@@ -28,7 +28,7 @@ EXPORTS = (
     "roift_weights_dev", "roift_oift_fmax_dev", "roift_oift_edges_dev", "roift_relax_dev",
     "roift_pipeline_dev", "roift_oift_fmax", "roift_relax", "oift_fmax_gpu_run",
     "roift_relax_binary_dev", "roift_oift_gradweight_dev", "roift_relax_gradweight_dev", "roift_relax_rbf_dev",
-    "roift_relax_binary", "roift_batch_pipeline_dev",
+    "roift_relax_binary", "roift_batch_pipeline_dev", "roift_relax_adjoint_dev",
 )
 
 
@@ -81,6 +81,8 @@ def lib():
         L.roift_relax_rbf_dev.argtypes = [vp, pp, vp, ip, fl, vp, ip, vp, ip, vp, fp]
         L.roift_relax_binary.argtypes = [vp, pp, vp, vp, vp, vp, vp, vp, ip, vp, ip, vp]
         L.roift_batch_pipeline_dev.argtypes = [vp, ip, i32p, fl, fl, vp, ip, vp, i32p, vp, ip, vp, ip]
+        L.roift_relax_adjoint_dev.argtypes = [vp, pp, vp, ip, vp, ip, vp, ip, ip, ip, vp, fl, fl, vp, fl, dp, vp, dp,
+                                              dp, vp]
         _lib = L
     return _lib
 
@@ -308,6 +310,24 @@ class Context:
             out.append(None if cost is None else cost[o:o + k].view(t.shape))
             o += k
         return out
+
+    def relax_adjoint(self, image, seeds, label, lab_obj, niter, scale, sigma, alpha, target, ckpt_every=10,
+                      adj_r=1.0, eps=1e-6, want_memb=False):
+        """Differentiable relaxation + adjoint: returns (loss, grad_scale, grad_sigma, grad_alpha, f or None)."""
+        import torch
+        pb, _ = make_problem(image.shape, adj_r, 0.0)
+        seeds = seeds.to(device=label.device, dtype=torch.int32).contiguous()
+        sc = scale.to(torch.float32).contiguous()
+        tg = target.to(torch.float32).contiguous()
+        gsc = torch.empty_like(sc)
+        memb = torch.empty_like(sc) if want_memb else None
+        loss, gs, ga = C.c_double(), C.c_double(), C.c_double()
+        self._check(self._L.roift_relax_adjoint_dev(
+            self._h, C.byref(pb), image.data_ptr(), _dtype_code(image), seeds.data_ptr(), int(seeds.numel()),
+            label.data_ptr(), int(lab_obj), int(niter), int(ckpt_every), sc.data_ptr(), float(sigma), float(alpha),
+            tg.data_ptr(), float(eps), C.byref(loss), gsc.data_ptr(), C.byref(gs), C.byref(ga),
+            memb.data_ptr() if memb is not None else None))
+        return loss.value, gsc, gs.value, ga.value, memb
 
     def pipeline(self, image, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None, cost=None, memb=None):
         """weights + OIFT + relax + labels on device tensors (label in/out)."""

@@ -396,6 +396,22 @@ ROIFT_API int roift_batch_pipeline_dev(roift_ctx *ctx, int nvol, const int32_t *
     });
 }
 
+ROIFT_API int roift_relax_adjoint_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                      const int32_t *d_seeds, int n_seeds, const int32_t *d_label, int lab_obj,
+                                      int niter, int ckpt_every, const float *d_scale, float sigma, float alpha,
+                                      const float *d_target, float eps, double *h_loss, float *d_grad_scale,
+                                      double *h_grad_sigma, double *h_grad_alpha, float *d_memb)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        const int nadj = nadj_of(pb);
+        ROIFT_REQUIRE(image_dtype == ROIFT_I16 || image_dtype == ROIFT_I32, ROIFT_EINVAL, "image dtype must be I16/I32");
+        run_relax_diff(*ctx, d, nadj, d_image, image_dtype, d_seeds, n_seeds, d_label, lab_obj, niter, ckpt_every,
+                       d_scale, sigma, alpha, d_target, eps, h_loss, d_grad_scale, h_grad_sigma, h_grad_alpha, d_memb);
+        return ROIFT_OK;
+    });
+}
+
 ROIFT_API int roift_pipeline_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
                                  const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter, int32_t *d_cost,
                                  float *d_memb)

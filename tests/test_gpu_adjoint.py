@@ -1,0 +1,70 @@
+"""GPU: the checkpointed adjoint of the differentiable relaxation against the FP64 oracle's
+reverse-mode gradients and against FP64 central finite differences, plus checkpoint
+invariance.  FP32 states bound the agreement: memberships within 1e-5 absolute, loss within
+1e-5 relative, gradients within 2e-3 relative (of the largest entry for the per-voxel scale).
+The BCE clamp eps is 1e-3: with o = 1 - f near 0 the FP32 state's absolute error would be
+amplified by 1/o.)"""
+import numpy as np
+import pytest
+
+import oracle as O
+from roift_amd import phantom as P
+
+pytestmark = pytest.mark.gpu
+EPS = 1e-3
+
+
+def _setup(shape, seed, adj):
+    c = P.small_case(shape, seed, per=50.0, adj_r=adj, top=255)
+    lab = O.oift_image(c.image, c.seeds, c.label_init(), adj, 50.0)["label"]
+    rng = np.random.default_rng(seed)
+    scale = (1.0 + 0.2 * rng.standard_normal(c.image.shape)).astype(np.float32)
+    target = np.clip(0.1 + 0.8 * (lab > 0) + 0.05 * rng.standard_normal(c.image.shape), 0.01, 0.99)
+    return c, lab, scale, target.astype(np.float32)
+
+
+def _gpu(ctx, c, lab, scale, target, niter, sigma, alpha, adj, ckpt):
+    import torch
+    dev = torch.device("cuda")
+    return ctx.relax_adjoint(torch.from_numpy(c.image.astype(np.int16)).to(dev),
+                             torch.from_numpy(c.seeds.astype(np.int32)).to(dev),
+                             torch.from_numpy(lab.astype(np.int32)).to(dev), 1, niter,
+                             torch.from_numpy(scale).to(dev), sigma, alpha, torch.from_numpy(target).to(dev),
+                             ckpt_every=ckpt, adj_r=adj, eps=EPS, want_memb=True)
+
+
+@pytest.mark.parametrize("shape,seed,adj,niter", [((12, 14, 16), 21, 1.0, 25), ((10, 11, 12), 22, 1.75, 12)])
+def test_adjoint_matches_fp64_oracle_and_finite_differences(ctx, shape, seed, adj, niter):
+    c, lab, scale, target = _setup(shape, seed, adj)
+    sigma, alpha = 30.0, 0.4
+    print(f"case shape={shape} adj={adj} niter={niter}", flush=True)
+    loss, gsc, gs, ga, f = _gpu(ctx, c, lab, scale, target, niter, sigma, alpha, adj, 10)
+    r = O.relax_diff(c.image, c.seeds, lab, 1, niter, scale.astype(np.float64), sigma, alpha,
+                     target.astype(np.float64), adj, eps=EPS)
+    assert np.abs(f.cpu().numpy() - r["f"]).max() <= 1e-5
+    assert abs(loss - r["loss"]) <= 1e-5 * abs(r["loss"])
+    assert abs(gs - r["g_sigma"]) <= 2e-3 * abs(r["g_sigma"])
+    assert abs(ga - r["g_alpha"]) <= 2e-3 * abs(r["g_alpha"])
+    g = gsc.cpu().numpy().astype(np.float64)
+    assert np.abs(g - r["g_scale"]).max() <= 2e-3 * np.abs(r["g_scale"]).max()
+    # FP64 central differences (h = 1e-4) of the loss itself
+    h = 1e-4
+
+    def L(s=sigma, a=alpha):
+        return O.relax_diff(c.image, c.seeds, lab, 1, niter, scale.astype(np.float64), s, a,
+                            target.astype(np.float64), adj, eps=EPS)["loss"]
+
+    fd_s = (L(s=sigma + h) - L(s=sigma - h)) / (2 * h)
+    fd_a = (L(a=alpha + h) - L(a=alpha - h)) / (2 * h)
+    assert abs(gs - fd_s) <= 2e-3 * abs(fd_s)
+    assert abs(ga - fd_a) <= 2e-3 * abs(fd_a)
+
+
+def test_checkpoint_interval_does_not_change_the_gradient(ctx):
+    c, lab, scale, target = _setup((12, 13, 14), 23, 1.0)
+    ref = _gpu(ctx, c, lab, scale, target, 20, 25.0, 0.3, 1.0, 1)
+    for ck in (3, 10, 20, 50):
+        got = _gpu(ctx, c, lab, scale, target, 20, 25.0, 0.3, 1.0, ck)
+        assert got[0] == pytest.approx(ref[0], rel=1e-12)
+        assert np.array_equal(got[1].cpu().numpy(), ref[1].cpu().numpy())
+        assert got[2] == pytest.approx(ref[2], rel=1e-9) and got[3] == pytest.approx(ref[3], rel=1e-9)
