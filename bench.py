@@ -210,6 +210,7 @@ def main():
         out["traffic"] = int(tr * n) if tr is not None else None
         return out
 
+    prof = ctx.oift_profile()
     oift_k = float(np.mean(kern["oift_kernel_ms"]))
     relax_k = float(np.mean(kern["relax_loop_ms"])) / case.niter
     weights_k = float(np.mean(kern["weights_ms"]))
@@ -232,8 +233,11 @@ def main():
         "stages_ms": {k: round(float(np.mean(v)), 3) for k, v in stage.items() if isinstance(v, list)},
         "kernel_ms": {"weights": round(weights_k, 4), "oift_persistent": round(oift_k, 3),
                       "relax_iteration": round(relax_k, 4)},
-        "oift_phase_ms": {k: v[0] for k, v in ctx.oift_profile().items()},
-        "oift_phase_count": {k: v[1] for k, v in ctx.oift_profile().items()},
+        "oift_phase_ms": {k: v[0] for k, v in prof.items() if k in ctx.PHASES},
+        "oift_phase_count": {k: v[1] for k, v in prof.items() if k in ctx.PHASES},
+        "oift_flood_items": prof["flood_items"][0],
+        "oift_touched_slots": prof["touched_slots"][0],
+        "oift_flood_batches": {"sum": prof["flood_batches"][0], "max_per_workgroup": prof["flood_batches"][1]},
         "oift_level_steps": stage["oift_steps"],
         "oift_flood_rounds": stage["flood_rounds"],
         "pipeline_roofline": {"bound": "hbm", "algorithmic_bytes_per_voxel": pipe_bytes // n,

@@ -70,7 +70,10 @@ ROIFT_API int roift_kernel_timings(const roift_ctx *ctx, double *out, int n);
    level-scan, collect, key-collect, key-rank, flood-seed, flood-round,
    r_safe, claim, offers; out[9..17] how many times each phase ran; with
    n >= 162, out[18 + 16*phase + b] splits each phase's time by the step's
-   entry count (bucket b = ceil(log2(entries)), capped at 15).  n >= 18.      */
+   entry count (bucket b = ceil(log2(entries)), capped at 15); with n >= 164,
+   out[162] = items the flood rounds processed, out[163] = touched-list slots
+   the claims scanned; with n >= 166, out[164] / out[165] = flood batches summed
+   over / maximal among the workgroups.  n >= 18.                               */
 ROIFT_API int roift_oift_profile(const roift_ctx *ctx, double *out, int n);
 
 /* Tuning knobs (the autotuner in roift_amd/tune.py sets them):

@@ -162,9 +162,12 @@ class Context:
     def oift_profile(self, by_size=False):
         """Per-phase device wall-clock ms and counts of the last OIFT (and, with
         by_size, each phase's ms split by ceil(log2(entries of the step)))."""
-        out = (C.c_double * 162)()
-        self._check(self._L.roift_oift_profile(self._h, out, 162))
+        out = (C.c_double * 166)()
+        self._check(self._L.roift_oift_profile(self._h, out, 166))
         res = {ph: (round(out[i], 3), int(out[9 + i])) for i, ph in enumerate(self.PHASES)}
+        res["flood_items"] = (int(out[162]), 0)
+        res["touched_slots"] = (int(out[163]), 0)
+        res["flood_batches"] = (int(out[164]), int(out[165]))
         if by_size:
             res = {ph: [round(out[18 + 16 * i + b], 2) for b in range(16)] for i, ph in enumerate(self.PHASES)}
         return res

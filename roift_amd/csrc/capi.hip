@@ -210,7 +210,7 @@ ROIFT_API int roift_set_param(roift_ctx *c, int param, int value)
 ROIFT_API int roift_oift_profile(const roift_ctx *c, double *out, int n)
 {
     if (!c || !out || n < 18) return ROIFT_EINVAL;
-    for (int i = 0; i < n && i < 162; i++) out[i] = c->prof[i];
+    for (int i = 0; i < n && i < 166; i++) out[i] = c->prof[i];
     return ROIFT_OK;
 }
 

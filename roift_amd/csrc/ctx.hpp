@@ -14,7 +14,8 @@ struct roift_ctx {
     std::map<std::string, std::pair<void *, size_t>> bufs;
     uint32_t *h_ctr = nullptr;      // pinned host mirror of the device counters
     double timings[6] = {0, 0, 0, 0, 0, 0};
-    double prof[162] = {};          // OIFT phase ms [0..8], counts [9..17], ms by log2(entries) [18..161]
+    double prof[166] = {};          // OIFT phase ms [0..8], counts [9..17], ms by log2(entries) [18..161],
+                                    // flood items [162], touched slots [163], batches sum [164] / max per workgroup [165]
     int tune_blocks = 0;            // OIFT persistent grid size (0: one workgroup per CU)
     int tune_budget = 0;            // OIFT LDS batches per workgroup per flood round (0: default)
     int tune_relax_depth = 0;       // relaxation temporal-block depth (0: default)

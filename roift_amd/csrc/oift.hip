@@ -106,6 +106,8 @@ struct State {
 };
 
 constexpr int NPHASE = 9;   // L1 L2 S1 S2 I1 R I2 I3 S3
+// counters in the profile block: flood items and batches processed, touched-list slots claimed
+constexpr int PROF_ITEMS = 240, PROF_TOUCHED = 241, PROF_BATCHES = 242, PROF_BATCHES_MAX = 243;
 
 __device__ inline int label_class(const State &s, int lb)
 {
