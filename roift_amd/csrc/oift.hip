@@ -257,7 +257,9 @@ int run_oift(roift_ctx &c, const Dims &d, int nadj, const void *d_wa, const void
     s.E[1] = buf<uint32_t>(c, "oift.E1", n);
     s.W[0] = buf<uint32_t>(c, "oift.W0", n);
     s.W[1] = buf<uint32_t>(c, "oift.W1", n);
-    s.T = buf<uint32_t>(c, "oift.T", n + (long long)MAX_BLOCKS * (BLOCK / 64) * TCHUNK);
+    // a voxel is listed once per class that reaches it: at most 2n entries, plus the
+    // per-wavefront chunk slack
+    s.T = buf<uint32_t>(c, "oift.T", 2 * n + (long long)MAX_BLOCKS * (BLOCK / 64) * TCHUNK);
     s.NS = buf<uint32_t>(c, "oift.NS", n);
     s.D = buf<uint32_t>(c, "oift.D", n);
     s.ctr = buf<uint32_t>(c, "oift.ctr", K_COUNT);

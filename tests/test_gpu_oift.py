@@ -223,3 +223,21 @@ def test_gft_mirror_api(ctx):
     r64 = O.relax(case["image"], case["seeds"], case["oift_label"], case["niter"], 1.0, case["per"], fp64=True)
     near = np.abs(r64["memb"][0] - 0.5) < 1e-5
     assert np.array_equal(label.volume()[~near], case["relax_label"].astype(np.int32)[~near])
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c[2] == 1.0][:5] + [("headline", 96)])
+def test_oift_brick_flood_mode_bit_exact(ctx, case):
+    """The optional 8x8x8 brick flood (ROIFT_PARAM_FLOOD_MODE = 1) gives the same forest."""
+    if case[0] == "headline":
+        c = P.headline_case(case[1])
+        adj, per = 1.0, 50.0
+    else:
+        shape, seed, adj, per, top, _ = case
+        c = P.small_case(shape, seed, per=per, adj_r=adj, top=top)
+    ctx.set_param("flood_mode", 1)
+    try:
+        lab, cost = run_gpu(ctx, c.image, c.seeds, c.label_init(), adj, per)
+    finally:
+        ctx.set_param("flood_mode", 0)
+    o = O.oift_image(c.image, c.seeds, c.label_init(), adj, per)
+    assert np.array_equal(lab, o["label"]) and np.array_equal(cost, o["cost"])
