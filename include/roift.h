@@ -81,12 +81,15 @@ ROIFT_API int roift_oift_profile(const roift_ctx *ctx, double *out, int n);
    ROIFT_PARAM_FLOOD_BUDGET   LDS batches a workgroup relaxes per flood round before spilling
    ROIFT_PARAM_RELAX_DEPTH    relaxation iterations fused per pass (temporal-block depth)
    ROIFT_PARAM_FLOOD_MODE     OIFT flood unit: 0 = single voxels (default), 1 = 8x8x8
-                              bricks relaxed in LDS (6-adjacency only; slower on thin fronts) */
+                              bricks relaxed in LDS (6-adjacency only; slower on thin fronts)
+   ROIFT_PARAM_RELAX_TILE     two-class 6-adjacency relaxation tile: rows (1..8) | planes << 8
+                              | xcd << 16 (1: neighbouring tiles on one XCD); 0 = default   */
 enum roift_param {
     ROIFT_PARAM_OIFT_BLOCKS = 1,
     ROIFT_PARAM_FLOOD_BUDGET = 2,
     ROIFT_PARAM_RELAX_DEPTH = 3,
-    ROIFT_PARAM_FLOOD_MODE = 4
+    ROIFT_PARAM_FLOOD_MODE = 4,
+    ROIFT_PARAM_RELAX_TILE = 5
 };
 ROIFT_API int roift_set_param(roift_ctx *ctx, int param, int value);
 

@@ -151,7 +151,7 @@ class Context:
         return {"weights_ms": out[0], "oift_kernel_ms": out[1], "relax_loop_ms": out[2],
                 "relax_launches": int(out[3])}
 
-    PARAMS = {"oift_blocks": 1, "flood_budget": 2, "relax_depth": 3, "flood_mode": 4}
+    PARAMS = {"oift_blocks": 1, "flood_budget": 2, "relax_depth": 3, "flood_mode": 4, "relax_tile": 5}
 
     def set_param(self, name, value):
         self._check(self._L.roift_set_param(self._h, self.PARAMS[name], int(value)))

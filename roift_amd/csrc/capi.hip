@@ -204,6 +204,7 @@ ROIFT_API int roift_set_param(roift_ctx *c, int param, int value)
     case ROIFT_PARAM_FLOOD_BUDGET: c->tune_budget = value; return ROIFT_OK;
     case ROIFT_PARAM_RELAX_DEPTH: c->tune_relax_depth = value; return ROIFT_OK;
     case ROIFT_PARAM_FLOOD_MODE: c->tune_flood_mode = value; return ROIFT_OK;
+    case ROIFT_PARAM_RELAX_TILE: c->tune_relax_tile = value; return ROIFT_OK;
     default: return ROIFT_EINVAL;
     }
 }

@@ -20,6 +20,7 @@ struct roift_ctx {
     int tune_budget = 0;            // OIFT LDS batches per workgroup per flood round (0: default)
     int tune_relax_depth = 0;       // relaxation temporal-block depth (0: default)
     int tune_flood_mode = 0;        // OIFT flood: 0 = single voxels, 1 = 8x8x8 bricks (6-adjacency)
+    int tune_relax_tile = 0;        // z-marching relaxation tile: rows | planes << 8 | xcd << 16 (0: default)
     hipEvent_t ev[16] = {};
     hipGraphExec_t relax_graph = nullptr;   // batched relaxation iterations, replayed while the batch is unchanged
     std::string relax_graph_key;

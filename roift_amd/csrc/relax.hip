@@ -43,6 +43,10 @@ struct RelaxArgs {
     const int32_t *G;
     float alpha;
     float sigma2;
+    // z-marching tiles: rows (blockDim.y) and planes per workgroup; xcd = 1 maps consecutive
+    // tiles to one XCD (workgroups are dealt to the 8 XCDs round-robin), so the tiles that
+    // share halo rows share an L2
+    int my, mz, xcd;
 };
 
 // Weight of neighbour q in p's update under the three relaxation models
@@ -240,7 +244,7 @@ __global__ void k_memb_out2(long long n, const float *__restrict__ f, int bg, fl
 // marches a chunk of z planes, keeping the z-1 / z / z+1 values in registers and
 // reading its x/y neighbours (L1/L2-resident) directly.  The image is int16 with the
 // seed flag in bit 15 (intensities are 0..32767); seeds keep their membership.
-constexpr int MX = 64, MY = 4, MZ = 32;
+constexpr int MX = 64, MY = 4, MZ = 32, MY_MAX = 8;
 
 // Weight of neighbour q in p's update, restated for fewer operations:
 //   |d| * (1 +- s/100) = |d| + d*t  with d = I_p - I_q and t = s/100 = tA*f_q + tB
@@ -269,7 +273,7 @@ struct MarchVol {
 };
 
 template <bool BATCH>
-__global__ __launch_bounds__(MX *MY) void k_relax2_march(const int16_t *__restrict__ img, RelaxArgs a,
+__global__ __launch_bounds__(MX *MY_MAX) void k_relax2_march(const int16_t *__restrict__ img, RelaxArgs a,
                                                           const MarchTile *__restrict__ tiles,
                                                           const MarchVol *__restrict__ vols,
                                                           const float *__restrict__ fin, float *__restrict__ fout,
@@ -292,34 +296,43 @@ __global__ __launch_bounds__(MX *MY) void k_relax2_march(const int16_t *__restri
         xs = a.d.xs;
         ys = a.d.ys;
         zs = a.d.zs;
-        x = blockIdx.x * MX + threadIdx.x;
-        y = blockIdx.y * MY + threadIdx.y;
-        z0 = blockIdx.z * MZ;
+        const unsigned ntx = (unsigned)(xs + MX - 1) / MX, nty = (unsigned)(ys + a.my - 1) / a.my;
+        const unsigned ntiles = ntx * nty * ((unsigned)(zs + a.mz - 1) / a.mz);
+        unsigned tile = blockIdx.x;
+        if (a.xcd) tile = (blockIdx.x % 8u) * ((ntiles + 7u) / 8u) + blockIdx.x / 8u;
+        if (tile >= ntiles) return;   // no block-level synchronisation below
+        x = (int)(tile % ntx) * MX + (int)threadIdx.x;
+        y = (int)((tile / ntx) % nty) * a.my + (int)threadIdx.y;
+        z0 = (int)(tile / (ntx * nty)) * a.mz;
     }
-    const int z1 = min(zs, z0 + MZ);
+    const int z1 = min(zs, z0 + (BATCH ? MZ : a.mz));
     if (x >= xs || y >= ys || z0 >= zs) return;   // no block-level synchronisation below
-    const int xy = xs * ys;
+    const uint32_t xy = (uint32_t)(xs * ys), uxs = (uint32_t)xs;
     img += base;
     fin += base;
     fout += base;
     if (label_out) label_out += base;
-    int p = x + y * xs + z0 * xy;
+    // 32-bit element offsets from a uniform base: the loads take a scalar base and a
+    // 32-bit vector offset, with no 64-bit address arithmetic per neighbour
+    auto LI = [&](uint32_t i) -> int { return *(const int16_t *)((const char *)img + (size_t)(i << 1)); };
+    auto LF = [&](uint32_t i) -> float { return *(const float *)((const char *)fin + (size_t)(i << 2)); };
+    uint32_t p = (uint32_t)x + (uint32_t)y * uxs + (uint32_t)z0 * xy;
     const int lab_bg = a.cls_label[a.bg], lab_ob = a.cls_label[1 - a.bg];
     bool hm = z0 > 0;
     float Im = 0.0f, fm = 0.0f;
     if (hm) {
-        Im = (float)(img[p - xy] & 0x7FFF);
-        fm = fin[p - xy];
+        Im = (float)(LI(p - xy) & 0x7FFF);
+        fm = LF(p - xy);
     }
-    int rc = img[p];
-    float fc = fin[p];
+    int rc = LI(p);
+    float fc = LF(p);
     for (int z = z0; z < z1; z++, p += xy) {
         const bool hp = z + 1 < zs;
         int rn = 0;
         float fn = 0.0f;
         if (hp) {
-            rn = img[p + xy];
-            fn = fin[p + xy];
+            rn = LI(p + xy);
+            fn = LF(p + xy);
         }
         const float Ic = (float)(rc & 0x7FFF);
         float v = fc;
@@ -332,26 +345,26 @@ __global__ __launch_bounds__(MX *MY) void k_relax2_march(const int16_t *__restri
                 acc = fmaf(w, fm, acc);
             }
             if (y > 0) {
-                const float fq = fin[p - xs];
-                w = relax_w8(Ic - (float)(img[p - xs] & 0x7FFF), fq, a);
+                const float fq = LF(p - uxs);
+                w = relax_w8(Ic - (float)(LI(p - uxs) & 0x7FFF), fq, a);
                 sw += w;
                 acc = fmaf(w, fq, acc);
             }
             if (x > 0) {
-                const float fq = fin[p - 1];
-                w = relax_w8(Ic - (float)(img[p - 1] & 0x7FFF), fq, a);
+                const float fq = LF(p - 1);
+                w = relax_w8(Ic - (float)(LI(p - 1) & 0x7FFF), fq, a);
                 sw += w;
                 acc = fmaf(w, fq, acc);
             }
             if (x < xs - 1) {
-                const float fq = fin[p + 1];
-                w = relax_w8(Ic - (float)(img[p + 1] & 0x7FFF), fq, a);
+                const float fq = LF(p + 1);
+                w = relax_w8(Ic - (float)(LI(p + 1) & 0x7FFF), fq, a);
                 sw += w;
                 acc = fmaf(w, fq, acc);
             }
             if (y < ys - 1) {
-                const float fq = fin[p + xs];
-                w = relax_w8(Ic - (float)(img[p + xs] & 0x7FFF), fq, a);
+                const float fq = LF(p + uxs);
+                w = relax_w8(Ic - (float)(LI(p + uxs) & 0x7FFF), fq, a);
                 sw += w;
                 acc = fmaf(w, fq, acc);
             }
@@ -363,8 +376,8 @@ __global__ __launch_bounds__(MX *MY) void k_relax2_march(const int16_t *__restri
             }
             v = (sw > 0.0f) ? acc / sw : fc;
         }
-        fout[p] = v;
-        if (label_out) label_out[p] = ((1.0f - v) > v) ? lab_ob : lab_bg;
+        *(float *)((char *)fout + (size_t)(p << 2)) = v;
+        if (label_out) *(int32_t *)((char *)label_out + (size_t)(p << 2)) = ((1.0f - v) > v) ? lab_ob : lab_bg;
         hm = true;
         Im = Ic;
         fm = fc;
@@ -658,12 +671,24 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
             hipLaunchKernelGGL(k_pack_image<int32_t>, dim3(G), dim3(256), 0, st, (const int32_t *)d_image, seedcls, n, pimg);
         ROIFT_HIP_CHECK(hipGetLastError());
     }
-    const dim3 mgrid((d.xs + MX - 1) / MX, (d.ys + MY - 1) / MY, (d.zs + MZ - 1) / MZ);
+    // tile shape: ROIFT_PARAM_RELAX_TILE = rows | planes << 8 | xcd << 16 (0: 4 rows, 32 planes, XCD-grouped)
+    a.my = MY;
+    a.mz = MZ;
+    a.xcd = 1;
+    if (c.tune_relax_tile > 0) {
+        a.my = std::min(MY_MAX, std::max(1, c.tune_relax_tile & 0xFF));
+        a.mz = std::max(1, (c.tune_relax_tile >> 8) & 0xFF);
+        a.xcd = (c.tune_relax_tile >> 16) & 1;
+    }
+    const unsigned ntiles = (unsigned)((d.xs + MX - 1) / MX) * (unsigned)((d.ys + a.my - 1) / a.my) *
+                            (unsigned)((d.zs + a.mz - 1) / a.mz);
+    const dim3 mgrid(a.xcd ? 8u * ((ntiles + 7u) / 8u) : ntiles);
     ev_record(c, 12);
     for (int it = 0; it < niter; it++) {
         int32_t *lo = (it == niter - 1) ? d_label : nullptr;
         if (march) {
-            hipLaunchKernelGGL(k_relax2_march<false>, mgrid, dim3(MX, MY), 0, st, pimg, a, nullptr, nullptr, fin, fout, lo);
+            hipLaunchKernelGGL(k_relax2_march<false>, mgrid, dim3(MX, a.my), 0, st, pimg, a, nullptr, nullptr, fin, fout,
+                               lo);
             ROIFT_HIP_CHECK(hipGetLastError());
         } else if (nadj == 7) {
             if (i16) launch_iter<7, int16_t>(c, d_image, a, rm.model, fin, fout, seedcls, lo);
