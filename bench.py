@@ -6,7 +6,8 @@ labels (roift_pipeline_dev), preceded by restoring the seed-only label map.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--no-cpu] [--no-batch] [--no-adjoint]
 
-Prints one JSON line.  Its "batched" object times the 64-volume batch (volumes of
+Prints one JSON line.  Its "blur" object times the CLI's preprocessing (two Gaussian-blur
+passes, one fused launch) on the same volume, outside the step.  Its "batched" object times the 64-volume batch (volumes of
 64-192 voxels a side) as 64 plain calls and as one batched call; its "adjoint" object
 times the differentiable relaxation's forward + checkpointed adjoint on 16 x 128^3.  The
 CPU baseline runs the reference's own
@@ -26,7 +27,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # algorithmic HBM bytes per voxel (int16 image, uint16 weights, FP32 single-channel memberships):
-ALG = dict(weights=2 + 3 * 2 * 2, oift=12 + 4 + 4 + 4, relax_iter=2 + 4 + 4, labels=4)
+ALG = dict(weights=2 + 3 * 2 * 2, oift=12 + 4 + 4 + 4, relax_iter=2 + 4 + 4, labels=4,
+           blur2=2 + 2)   # both Gaussian-blur passes fused: read the int16 image once, write it once
 # HBM bytes per voxel per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
 # workload, scaled by the same counters on a 1 GiB copy (profiles/pmc_traffic_headline256.txt).
 TRAFFIC = {"k_oift_persistent": 2704.9, "k_relax2_march": 19.74, "k_weights": 16.13}
@@ -156,7 +158,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "volumes shard across GPUs as described in DESIGN.md but no multi-GPU path is built.",
+              "batches would shard across GPUs by volume, but no multi-GPU path is built.",
               file=sys.stderr)
         sys.exit(2)
 
@@ -211,6 +213,14 @@ def main():
         return out
 
     prof = ctx.oift_profile()
+    # the CLI's preprocessing (oiftrelax.cpp:368-375): two Gaussian-blur passes, outside the step
+    blurred = torch.empty_like(img)
+    blur_ms = []
+    for i in range(6):
+        ctx.gaussian_blur(img, passes=2, out=blurred)
+        if i:
+            blur_ms.append(ctx.kernel_timings()["blur_ms"])
+    blur_k = float(np.mean(blur_ms))
     oift_k = float(np.mean(kern["oift_kernel_ms"]))
     relax_k = float(np.mean(kern["relax_loop_ms"])) / case.niter
     weights_k = float(np.mean(kern["weights_ms"]))
@@ -247,7 +257,10 @@ def main():
         # the dominant kernel (most of the step's time): the persistent OIFT kernel
         "roofline": roof("k_oift_persistent", ALG["oift"], oift_k),
         "other_rooflines": [roof("k_relax2_march", ALG["relax_iter"], relax_k),
-                            roof("k_weights", ALG["weights"], weights_k)],
+                            roof("k_weights", ALG["weights"], weights_k),
+                            roof("k_blur", ALG["blur2"], blur_k)],
+        "blur": {"passes": 2, "launches": 1, "ms": round(blur_k, 4),
+                 "Mvoxels_per_s": round(n / (blur_k / 1e3) / 1e6, 1)},
     }
     if not args.no_batch:
         res["batched"] = batched_bench(ctx)

@@ -62,7 +62,8 @@ ROIFT_API int roift_set_stream(roift_ctx *ctx, void *hip_stream);
 ROIFT_API int roift_last_timings(const roift_ctx *ctx, double *out6);
 /* Device-event times of the last call's main kernels, in milliseconds:
    [0] weight builder, [1] OIFT persistent kernel, [2] all relaxation iterations
-   (back-to-back launches), [3] number of relaxation launches.  n >= 4.
+   (back-to-back launches), [3] number of relaxation launches; with n >= 5,
+   [4] the last Gaussian blur (all passes).  n >= 4.
    (No reference counterpart: measurement only.) */
 ROIFT_API int roift_kernel_timings(const roift_ctx *ctx, double *out, int n);
 /* Wall-clock breakdown of the last OIFT's persistent kernel, measured on the
@@ -83,13 +84,15 @@ ROIFT_API int roift_oift_profile(const roift_ctx *ctx, double *out, int n);
    ROIFT_PARAM_FLOOD_MODE     OIFT flood unit: 0 = single voxels (default), 1 = 8x8x8
                               bricks relaxed in LDS (6-adjacency only; slower on thin fronts)
    ROIFT_PARAM_RELAX_TILE     two-class 6-adjacency relaxation tile: rows (1..8) | planes << 8
-                              | xcd << 16 (1: neighbouring tiles on one XCD); 0 = default   */
+                              | xcd << 16 (1: neighbouring tiles on one XCD); 0 = default
+   ROIFT_PARAM_BLUR_PLANES    Gaussian blur: z planes per workgroup (0 = automatic)         */
 enum roift_param {
     ROIFT_PARAM_OIFT_BLOCKS = 1,
     ROIFT_PARAM_FLOOD_BUDGET = 2,
     ROIFT_PARAM_RELAX_DEPTH = 3,
     ROIFT_PARAM_FLOOD_MODE = 4,
-    ROIFT_PARAM_RELAX_TILE = 5
+    ROIFT_PARAM_RELAX_TILE = 5,
+    ROIFT_PARAM_BLUR_PLANES = 6
 };
 ROIFT_API int roift_set_param(roift_ctx *ctx, int param, int value);
 
@@ -208,6 +211,17 @@ ROIFT_API int roift_relax_adjoint_dev(roift_ctx *ctx, const roift_problem *pb, c
                                       const float *d_target, float eps, double *h_loss, float *d_grad_scale,
                                       double *h_grad_sigma, double *h_grad_alpha, float *d_memb);
 
+/* ---- preprocessing ------------------------------------------------------------ */
+/* gft::Scene32::GaussianBlur (gft_filtering3.cpp:385-397) applied `passes` times
+ * (the CLI blurs twice before the OIFT, oiftrelax.cpp:368-375): 7-tap spherical
+ * Gaussian of radius 1 (Kernel3::SphericalGaussian(scn, 1, 10, 1) normalised,
+ * gft_filtering3.cpp:72-126), taps outside the volume skipped, FP32 sum in the
+ * reference's tap order and gft's ROUND, so the output is bit-exact.  Only
+ * pb->xsize/ysize/zsize are read.  d_in and d_out hold N values of image_dtype
+ * (I16 or I32; the output has the input's dtype) and may be the same buffer.     */
+ROIFT_API int roift_gaussian_blur_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_in, int image_dtype,
+                                      void *d_out, int passes);
+
 /* Host-pointer forms (copy in, run, copy out).  h_image is int32 as gft's Scene32. */
 ROIFT_API int roift_oift_fmax(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image,
                               const int32_t *h_seeds, int n_seeds, int32_t *h_label, int32_t *h_cost);
@@ -216,6 +230,8 @@ ROIFT_API int roift_relax(roift_ctx *ctx, const roift_problem *pb, const int32_t
 ROIFT_API int roift_relax_binary(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image, const int32_t *h_W,
                                  const int32_t *h_Wx, const int32_t *h_Wy, const int32_t *h_Wz,
                                  const int32_t *h_seeds, int n_seeds, int32_t *h_label, int niter, float *h_memb);
+ROIFT_API int roift_gaussian_blur(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_in, int32_t *h_out,
+                                  int passes);
 
 /* ---- legacy reference C-ABI ---------------------------------------------------------- */
 /* Same symbol and signature as the reference's GPU entry point

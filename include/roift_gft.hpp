@@ -7,6 +7,7 @@
 //   gft::ift::ORelax_1_Multi          (gft_ift.h:451-456)
 //   gft::ift::ORelax_1_Multi_PerClass (gft_ift.h:459-465)
 //   gft::ift::ORelax_1 and its W / Wx,Wy,Wz forms (gft_ift.h:443-482)
+//   gft::Scene32::GaussianBlur        (gft_filtering3.h:54), the CLI's preprocessing
 // Each copies the scene to the GPU, runs libroift.so (include/roift.h) and writes
 // the label scene back in place, exactly as the originals modify `label`.
 //
@@ -43,6 +44,9 @@ struct sScene32 {
     int maxval, minval, n;
     void *nii_hdr;
 };
+// gft's constructor (gft_scene32.h), defined by libgft, not by libroift_gft.so
+sScene32 *Create(int xsize, int ysize, int zsize);
+sScene32 *GaussianBlur(sScene32 *scn);
 }  // namespace Scene32
 
 typedef AdjRel3::sAdjRel3 sAdjRel3;

@@ -65,6 +65,8 @@ def lib():
         L.orc_gradweight_edges.argtypes = [i32p, i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                                            u32p, u32p]
         L.orc_estimate_sigma.argtypes = [i32p, C.c_int64]
+        L.orc_gauss_kernel.argtypes = [f32p]
+        L.orc_gaussian_blur.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_int, i32p]
         L.orc_relax_diff_f64.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_float, i32p, i32p, C.c_int, C.c_int,
                                          f64p, C.c_double, C.c_double, f64p, C.c_double, f64p, f64p, f64p, f64p,
                                          f64p]
@@ -97,6 +99,8 @@ def ref_lib():
         L.ref_relax_rbf.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, i32p, i32p,
                                     C.c_int]
         L.ref_estimate_sigma.argtypes = [i32p, C.c_int, C.c_int, C.c_int]
+        L.ref_gaussian_blur.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_int, i32p]
+        L.ref_gauss_kernel.argtypes = [f32p, C.c_int]
         L.ref_estimate_sigma.restype = C.c_float
         _REF = L
     return _REF
@@ -304,6 +308,22 @@ def relax_diff(I, seeds, label, lab_obj, niter, scale, sigma, alpha, target, adj
     return dict(loss=float(loss[0]), g_scale=gsc, g_sigma=float(gs[0]), g_alpha=float(ga[0]), f=f)
 
 
+def gauss_kernel():
+    v = np.zeros(7, np.float32)
+    n = lib().orc_gauss_kernel(_p(v, C.c_float))
+    return v[:n]
+
+
+def gaussian_blur(I, passes=1):
+    """gft::Scene32::GaussianBlur applied `passes` times (orc_gaussian_blur)."""
+    I = _i32(I)
+    xs, ys, zs = _dims(I)
+    out = np.empty_like(I)
+    rc = lib().orc_gaussian_blur(_p(I, C.c_int32), xs, ys, zs, int(passes), _p(out, C.c_int32))
+    assert rc == 0
+    return out
+
+
 # ---- the compiled reference (only where oracle/_ref was built) -------------------
 
 def ref_oift(I, seeds, label, adj_r=1.0, per=0.0, per_class=None):
@@ -382,3 +402,16 @@ def ref_relax_rbf(I, seeds, label, niter, sigma, adj_r=1.0, per=0.0):
 def ref_estimate_sigma(I):
     I = _i32(I); xs, ys, zs = _dims(I)
     return float(ref_lib().ref_estimate_sigma(_p(I, C.c_int32), xs, ys, zs))
+
+
+def ref_gaussian_blur(I, passes=1):
+    I = _i32(I); xs, ys, zs = _dims(I)
+    out = np.empty_like(I)
+    ref_lib().ref_gaussian_blur(_p(I, C.c_int32), xs, ys, zs, int(passes), _p(out, C.c_int32))
+    return out
+
+
+def ref_gauss_kernel():
+    v = np.zeros(8, np.float32)
+    n = ref_lib().ref_gauss_kernel(_p(v, C.c_float), 8)
+    return v[:n]

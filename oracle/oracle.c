@@ -926,3 +926,57 @@ int orc_relax_diff_f64(const int32_t *I, int xs, int ys, int zs, float adj_r, co
     free(F); free(lam); free(lam2); free(seed);
     return ORC_OK;
 }
+
+/* ------------------------------------------------------------------------ */
+/* gft::Scene32::GaussianBlur (gft_filtering3.cpp:385-397): convolution     */
+/* with Kernel3::SphericalGaussian(scn, R=1, s=10, f=1) normalised          */
+/* (gft_filtering3.cpp:72-126): taps in Spheric(1.0) order, centre first,   */
+/* val = (float)(s * exp((double)(-f * (r2 / R2)))) (exp of a double: the   */
+/* float argument is promoted), normalised by their float sum.  The         */
+/* convolution sums (float)I[q] * val in tap order over the taps inside the */
+/* volume (no renormalisation at the border) and rounds with gft's ROUND    */
+/* (gft_common.h:120, the 0.5 is a double) (gft_filtering3.cpp:321-346).    */
+/* ------------------------------------------------------------------------ */
+int orc_gauss_kernel(float *val)
+{
+    orc_adj A;
+    if (adj_make(1.0f, &A) != ORC_OK) return ORC_EINVAL;
+    const float R = 1.0f, s = 10.0f, f = 1.0f, R2 = R * R;
+    float wt = 0.0f;
+    for (int i = 0; i < A.n; i++) {
+        const float r2 = (float)(A.dx[i] * A.dx[i] + A.dy[i] * A.dy[i] + A.dz[i] * A.dz[i]);
+        val[i] = (float)((double)s * exp((double)(-f * (r2 / R2))));
+    }
+    for (int i = 0; i < A.n; i++) wt += val[i];
+    for (int i = 0; i < A.n; i++) val[i] /= wt;
+    return A.n;
+}
+
+int orc_gaussian_blur(const int32_t *I, int xs, int ys, int zs, int passes, int32_t *out)
+{
+    orc_adj A;
+    if (adj_make(1.0f, &A) != ORC_OK) return ORC_EINVAL;
+    float val[7];
+    orc_gauss_kernel(val);
+    const int64_t n = (int64_t)xs * ys * zs;
+    int32_t *tmp = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    if (!tmp) return ORC_ENOMEM;
+    memcpy(out, I, sizeof(int32_t) * (size_t)n);
+    for (int k = 0; k < passes; k++) {
+        memcpy(tmp, out, sizeof(int32_t) * (size_t)n);
+        for (int z = 0; z < zs; z++)
+            for (int y = 0; y < ys; y++)
+                for (int x = 0; x < xs; x++) {
+                    float conv = 0.0f;
+                    for (int i = 0; i < A.n; i++) {
+                        int qx = x + A.dx[i], qy = y + A.dy[i], qz = z + A.dz[i];
+                        if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) continue;
+                        conv += (float)tmp[qx + (int64_t)qy * xs + (int64_t)qz * xs * ys] * val[i];
+                    }
+                    out[x + (int64_t)y * xs + (int64_t)z * xs * ys] =
+                        (conv < 0) ? (int32_t)((double)conv - 0.5) : (int32_t)((double)conv + 0.5);
+                }
+    }
+    free(tmp);
+    return ORC_OK;
+}

@@ -110,6 +110,34 @@ REF_API int ref_relax_binary(const int *I, const int *W, const int *Wx, const in
     return 0;
 }
 
+// gft::Scene32::GaussianBlur (gft_filtering3.cpp:385-397) applied `passes` times, as the CLI
+// does before the OIFT (oiftrelax.cpp:368-375); and the normalized kernel it convolves with.
+REF_API int ref_gaussian_blur(const int *I, int xs, int ys, int zs, int passes, int *out)
+{
+    gft::sScene32 *scn = make_scene(I, xs, ys, zs);
+    for (int k = 0; k < passes; k++) {
+        gft::sScene32 *b = gft::Scene32::GaussianBlur(scn);
+        gft::Scene32::Destroy(&scn);
+        scn = b;
+    }
+    std::memcpy(out, scn->data, sizeof(int) * (size_t)scn->n);
+    gft::Scene32::Destroy(&scn);
+    return 0;
+}
+
+REF_API int ref_gauss_kernel(float *vals, int cap)
+{
+    gft::sScene32 *scn = gft::Scene32::Create(3, 3, 3);
+    gft::sKernel3 *K = gft::Kernel3::SphericalGaussian(scn, 1.0f, 10.0f, 1.0f);
+    gft::sKernel3 *NK = gft::Kernel3::Normalize(K);
+    const int n = NK->adj->n;
+    for (int i = 0; i < n && i < cap; i++) vals[i] = NK->val[i];
+    gft::Kernel3::Destroy(&NK);
+    gft::Kernel3::Destroy(&K);
+    gft::Scene32::Destroy(&scn);
+    return n;
+}
+
 // OIFT_Multi followed by ORelax_1_Multi on the same scenes, as oiftrelax.cpp:379-401
 // runs them; t_oift / t_relax receive steady_clock seconds around each call.
 REF_API int ref_pipeline(const int *I, int xs, int ys, int zs, float adj_r, float per,

@@ -29,6 +29,7 @@ EXPORTS = (
     "roift_pipeline_dev", "roift_oift_fmax", "roift_relax", "oift_fmax_gpu_run",
     "roift_relax_binary_dev", "roift_oift_gradweight_dev", "roift_relax_gradweight_dev", "roift_relax_rbf_dev",
     "roift_relax_binary", "roift_batch_pipeline_dev", "roift_relax_adjoint_dev",
+    "roift_gaussian_blur_dev", "roift_gaussian_blur",
 )
 
 
@@ -83,6 +84,8 @@ def lib():
         L.roift_batch_pipeline_dev.argtypes = [vp, ip, i32p, fl, fl, vp, ip, vp, i32p, vp, ip, vp, ip]
         L.roift_relax_adjoint_dev.argtypes = [vp, pp, vp, ip, vp, ip, vp, ip, ip, ip, vp, fl, fl, vp, fl, dp, vp, dp,
                                               dp, vp]
+        L.roift_gaussian_blur_dev.argtypes = [vp, pp, vp, ip, vp, ip]
+        L.roift_gaussian_blur.argtypes = [vp, pp, vp, vp, ip]
         _lib = L
     return _lib
 
@@ -145,13 +148,14 @@ class Context:
         return dict(zip(keys, list(out)))
 
     def kernel_timings(self):
-        """Device-event ms of the last call's main kernels (weights, OIFT, relaxation loop)."""
-        out = (C.c_double * 4)()
-        self._check(self._L.roift_kernel_timings(self._h, out, 4))
+        """Device-event ms of the last call's main kernels (weights, OIFT, relaxation loop, blur)."""
+        out = (C.c_double * 5)()
+        self._check(self._L.roift_kernel_timings(self._h, out, 5))
         return {"weights_ms": out[0], "oift_kernel_ms": out[1], "relax_loop_ms": out[2],
-                "relax_launches": int(out[3])}
+                "relax_launches": int(out[3]), "blur_ms": out[4]}
 
-    PARAMS = {"oift_blocks": 1, "flood_budget": 2, "relax_depth": 3, "flood_mode": 4, "relax_tile": 5}
+    PARAMS = {"oift_blocks": 1, "flood_budget": 2, "relax_depth": 3, "flood_mode": 4, "relax_tile": 5,
+              "blur_planes": 6}
 
     def set_param(self, name, value):
         self._check(self._L.roift_set_param(self._h, self.PARAMS[name], int(value)))
@@ -245,6 +249,27 @@ class Context:
                                                    int(seeds.numel()), label.data_ptr(), int(niter),
                                                    memb.data_ptr() if memb is not None else None))
         return memb
+
+    def gaussian_blur(self, image, passes=2, out=None):
+        """gft::Scene32::GaussianBlur applied `passes` times to an int16/int32 device volume.
+        Returns `out` (a new tensor of the input's dtype when None; may be `image` itself)."""
+        import torch
+        image = image.contiguous()
+        if out is None:
+            out = torch.empty_like(image)
+        if out.dtype != image.dtype or out.shape != image.shape or not out.is_contiguous():
+            raise ValueError("out must be a contiguous tensor of the image's dtype and shape")
+        pb, _ = make_problem(image.shape)
+        self._check(self._L.roift_gaussian_blur_dev(self._h, C.byref(pb), image.data_ptr(), _dtype_code(image),
+                                                    out.data_ptr(), int(passes)))
+        return out
+
+    def gaussian_blur_host(self, image, passes=2):
+        img = np.ascontiguousarray(image, np.int32)
+        out = np.empty_like(img)
+        pb, _ = make_problem(img.shape)
+        self._check(self._L.roift_gaussian_blur(self._h, C.byref(pb), img.ctypes.data, out.ctypes.data, int(passes)))
+        return out
 
     def oift_gradweight(self, image, grad, seeds, label, alpha=0.5, adj_r=1.0, per=0.0, want_cost=True):
         """OIFT_Multi_GradWeight in place on `label`; grad: int32 device tensor; returns the cost map."""

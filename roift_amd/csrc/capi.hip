@@ -193,6 +193,7 @@ ROIFT_API int roift_kernel_timings(const roift_ctx *c, double *out, int n)
 {
     if (!c || !out || n < 4) return ROIFT_EINVAL;
     for (int i = 0; i < 4; i++) out[i] = c->ktimes[i];
+    if (n >= 5) out[4] = c->ktimes[4];
     return ROIFT_OK;
 }
 
@@ -205,6 +206,7 @@ ROIFT_API int roift_set_param(roift_ctx *c, int param, int value)
     case ROIFT_PARAM_RELAX_DEPTH: c->tune_relax_depth = value; return ROIFT_OK;
     case ROIFT_PARAM_FLOOD_MODE: c->tune_flood_mode = value; return ROIFT_OK;
     case ROIFT_PARAM_RELAX_TILE: c->tune_relax_tile = value; return ROIFT_OK;
+    case ROIFT_PARAM_BLUR_PLANES: c->tune_blur_planes = value; return ROIFT_OK;
     default: return ROIFT_EINVAL;
     }
 }
@@ -508,6 +510,30 @@ ROIFT_API int roift_relax_binary(roift_ctx *ctx, const roift_problem *pb, const 
         if (rc != ROIFT_OK) return rc;
         ROIFT_HIP_CHECK(hipMemcpyAsync(h_label, lab, 4 * d.n, hipMemcpyDeviceToHost, ctx->stream));
         if (h_memb) ROIFT_HIP_CHECK(hipMemcpyAsync(h_memb, memb, 4 * d.n, hipMemcpyDeviceToHost, ctx->stream));
+        ROIFT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return ROIFT_OK;
+    });
+}
+
+ROIFT_API int roift_gaussian_blur_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_in, int image_dtype,
+                                      void *d_out, int passes)
+{
+    return guarded(ctx, [&]() -> int {
+        return run_gaussian_blur(*ctx, dims_of(pb), d_in, image_dtype, d_out, passes);
+    });
+}
+
+ROIFT_API int roift_gaussian_blur(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_in, int32_t *h_out,
+                                  int passes)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        ROIFT_REQUIRE(h_in && h_out, ROIFT_EINVAL, "null image pointer");
+        int32_t *img = buf<int32_t>(*ctx, "host.img", d.n);
+        ROIFT_HIP_CHECK(hipMemcpyAsync(img, h_in, 4 * d.n, hipMemcpyHostToDevice, ctx->stream));
+        const int rc = run_gaussian_blur(*ctx, d, img, ROIFT_I32, img, passes);
+        if (rc != ROIFT_OK) return rc;
+        ROIFT_HIP_CHECK(hipMemcpyAsync(h_out, img, 4 * d.n, hipMemcpyDeviceToHost, ctx->stream));
         ROIFT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return ROIFT_OK;
     });

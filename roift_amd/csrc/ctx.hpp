@@ -21,12 +21,13 @@ struct roift_ctx {
     int tune_relax_depth = 0;       // relaxation temporal-block depth (0: default)
     int tune_flood_mode = 0;        // OIFT flood: 0 = single voxels, 1 = 8x8x8 bricks (6-adjacency)
     int tune_relax_tile = 0;        // z-marching relaxation tile: rows | planes << 8 | xcd << 16 (0: default)
+    int tune_blur_planes = 0;       // Gaussian blur: planes per workgroup (0: automatic)
     hipEvent_t ev[16] = {};
     hipGraphExec_t relax_graph = nullptr;   // batched relaxation iterations, replayed while the batch is unchanged
     std::string relax_graph_key;
     // device-event times of the last call's main kernels, ms: weight builder, OIFT
-    // persistent kernel, all relaxation iterations; [3] = relaxation launches
-    double ktimes[4] = {0, 0, 0, 0};
+    // persistent kernel, all relaxation iterations; [3] = relaxation launches; [4] = Gaussian blur
+    double ktimes[5] = {0, 0, 0, 0, 0};
 };
 
 namespace roift {
@@ -134,6 +135,9 @@ struct BatchVolume {
 // Two-class 6-adjacency ORelax_1_Multi over a batch in one launch per iteration (labels
 // {0, lab_obj}); d_seeds are flat (batch-global) indices.  use_graph replays the
 // iteration launches as a HIP graph.
+// gft::Scene32::GaussianBlur applied `passes` times (in place when d_in == d_out).
+int run_gaussian_blur(roift_ctx &c, const Dims &d, const void *d_in, int dtype, void *d_out, int passes);
+
 int run_relax_batch(roift_ctx &c, const std::vector<BatchVolume> &vols, const void *d_image, int image_dtype,
                     float per, int lab_obj, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter,
                     bool use_graph);

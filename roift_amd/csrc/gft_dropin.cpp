@@ -1,7 +1,7 @@
 // gft_dropin.cpp -- gft-signature entry points over the C-ABI (libroift_gft.so).
 //
 // Same behaviour contract as the reference functions they replace
-// (gft_ift.cpp:911-1083, 8655-9187): `label` is modified in place; a NULL S
+// (gft_ift.cpp:911-1083, 8655-9187; GaussianBlur, gft_filtering3.cpp:385-397): `label` is modified in place; a NULL S
 // seeds every labelled voxel in raster order (gft_ift.cpp:944-948); where the
 // reference would call gft::Error (exit(-1)), this prints the library's error
 // and exits with the same status, since these void signatures cannot report it.
@@ -139,4 +139,21 @@ __attribute__((visibility("default"))) void ORelax_1(sScene32 *Wx, sScene32 *Wy,
 }
 
 }  // namespace ift
+
+namespace Scene32 {
+
+// The blurred scene is allocated by gft's own constructor (resolved from libgft at the
+// caller's link), so gft::Scene32::Destroy frees it as usual.
+__attribute__((visibility("default"))) sScene32 *GaussianBlur(sScene32 *scn)
+{
+    sScene32 *blur = Create(scn->xsize, scn->ysize, scn->zsize);
+    blur->dx = scn->dx;
+    blur->dy = scn->dy;
+    blur->dz = scn->dz;
+    const roift_problem pb{scn->xsize, scn->ysize, scn->zsize, 1.0f, 0, 0.0f, nullptr, -1};
+    check(roift_gaussian_blur(context(), &pb, scn->data, blur->data, 1));
+    return blur;
+}
+
+}  // namespace Scene32
 }  // namespace gft

@@ -2,6 +2,7 @@
 
 Same names, argument meaning and in-place behaviour as the C++ originals:
   gft::Scene32::Create / Destroy               (gft_scene32.h:32-40)
+  gft::Scene32::GaussianBlur                   (gft_filtering3.h:54, gft_filtering3.cpp:385-397)
   gft::AdjRel3::Spheric(r)                     (gft_adjrel3.cpp:47-85)
   gft::ift::OIFT_Multi / OIFT_Multi_PerClass   (gft_ift.h:108-122)
   gft::ift::ORelax_1_Multi / _PerClass         (gft_ift.h:451-465)
@@ -77,6 +78,12 @@ class Scene32:
     @staticmethod
     def GetMaximumValue(scn: sScene32) -> int:
         return int(scn.data.max())
+
+    @staticmethod
+    def GaussianBlur(scn: sScene32) -> sScene32:
+        """A new scene: the 7-tap spherical Gaussian of radius 1, computed on the GPU."""
+        out = _context().gaussian_blur_host(scn.volume(), passes=1)
+        return sScene32(out.ravel(), scn.xsize, scn.ysize, scn.zsize, scn.dx, scn.dy, scn.dz)
 
     @staticmethod
     def Destroy(scn) -> None:

@@ -2,13 +2,15 @@
 
 Runs gft::ift::OIFT_Multi[_PerClass] and ORelax_1_Multi[_PerClass] of the
 reference, and (into variants/) its ORelax_1 binary forms and the experiment
-variants OIFT_Multi_GradWeight / ORelax_1_Multi_GradWeight / ORelax_1_Multi_GaussRBF (oracle/_ref/libroift_ref.so, built by `make -C oracle ref` from the
-reference sources) on small deterministic inputs and stores inputs and
+variants OIFT_Multi_GradWeight / ORelax_1_Multi_GradWeight / ORelax_1_Multi_GaussRBF, and (into
+blur/) gft::Scene32::GaussianBlur applied 1-3 times (oracle/_ref/libroift_ref.so, built by
+`make -C oracle ref` from the reference sources) on small deterministic inputs and stores inputs and
 outputs as .npz.  The reference holds no fixtures of its own, so these are
 the reference-produced known answers the oracle and the GPU path are pinned
 to.  Re-run only if the inputs below change:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          # everything
+    python tests/golden/make_golden.py blur     # only blur/
 """
 import os
 import sys
@@ -134,9 +136,31 @@ def write_variants():
         print(name, img.shape, "obj voxels", int((oift > 0).sum()), int((relax > 0).sum()))
 
 
+def blur_cases():
+    rng = np.random.default_rng(77)
+    yield "ell", P.small_case((12, 14, 16), 11, per=50.0, top=255).image
+    yield "rand_signed", rng.integers(-5000, 70000, size=(9, 11, 13)).astype(np.int32)
+    yield "plane2d", rng.integers(0, 4096, size=(1, 40, 50)).astype(np.int32)
+    yield "thin", rng.integers(0, 30000, size=(3, 1, 70)).astype(np.int32)
+    yield "big_values", rng.integers(-(1 << 30), 1 << 30, size=(5, 6, 7)).astype(np.int32)
+
+
+def write_blur():
+    out_dir = os.path.join(HERE, "blur")
+    os.makedirs(out_dir, exist_ok=True)
+    for name, img in blur_cases():
+        img = np.ascontiguousarray(img, np.int32)
+        outs = {f"out{k}": O.ref_gaussian_blur(img, k) for k in (1, 2, 3)}
+        np.savez_compressed(os.path.join(out_dir, f"{name}.npz"), image=img, taps=O.ref_gauss_kernel(), **outs)
+        print("blur", name, img.shape)
+
+
 def main():
     if O.ref_lib() is None:
         raise SystemExit("oracle/_ref/libroift_ref.so is not built (make -C oracle ref)")
+    if sys.argv[1:] == ["blur"]:
+        write_blur()
+        return
     for name, c in cases():
         img = np.ascontiguousarray(c["image"], np.int32)
         lab0 = np.full(img.size, -1, np.int32)
@@ -152,6 +176,7 @@ def main():
             relax_label=relax.astype(np.int8))
         print(name, img.shape, "obj voxels", int((oift > 0).sum()), int((relax > 0).sum()))
     write_variants()
+    write_blur()
 
 
 if __name__ == "__main__":

@@ -44,6 +44,7 @@ GFT_SIGNATURES = {
     "gft::Scene32::sScene32*, int)",
     "gft::ift::ORelax_1(gft::Scene32::sScene32*, gft::Scene32::sScene32*, gft::Scene32::sScene32*, "
     "gft::Scene32::sScene32*, float, int*, gft::Scene32::sScene32*, int)",
+    "gft::Scene32::GaussianBlur(gft::Scene32::sScene32*)",
 }
 
 
@@ -58,9 +59,11 @@ def test_gft_dropin_exports_reference_signatures():
     signatures, so a caller links it ahead of -lgft unchanged."""
     syms = _defined(os.path.join(ROOT, "roift_amd", "libroift_gft.so"))
     assert GFT_SIGNATURES <= syms
-    ref_obj = os.path.join(ROOT, "oracle", "_ref", "obj", "gft_ift.cpp.o")
-    if os.path.exists(ref_obj):
-        assert GFT_SIGNATURES <= _defined(ref_obj, dynamic=False)
+    ref_dir = os.path.join(ROOT, "oracle", "_ref", "obj")
+    if os.path.exists(os.path.join(ref_dir, "gft_ift.cpp.o")):
+        ref = _defined(os.path.join(ref_dir, "gft_ift.cpp.o"), dynamic=False)
+        ref |= _defined(os.path.join(ref_dir, "gft_filtering3.cpp.o"), dynamic=False)
+        assert GFT_SIGNATURES <= ref
 
 
 def test_create_without_device_fails_cleanly():
