@@ -28,7 +28,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # algorithmic HBM bytes per voxel (int16 image, uint16 weights, FP32 single-channel memberships):
 ALG = dict(weights=2 + 3 * 2 * 2, oift=12 + 4 + 4 + 4, relax_iter=2 + 4 + 4, labels=4,
-           blur2=2 + 2)   # both Gaussian-blur passes fused: read the int16 image once, write it once
+           blur2=2 + 2,   # both Gaussian-blur passes fused: read the int16 image once, write it once
+           sobel=2 + 4)   # Sobel: read the int16 image, write the int32 gradient
 # HBM bytes per voxel per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
 # workload, scaled by the same counters on a 1 GiB copy (profiles/pmc_traffic_headline256.txt).
 TRAFFIC = {"k_oift_persistent": 2704.9, "k_relax2_march": 19.74, "k_weights": 16.13}
@@ -221,6 +222,30 @@ def main():
         if i:
             blur_ms.append(ctx.kernel_timings()["blur_ms"])
     blur_k = float(np.mean(blur_ms))
+    # the CLI's other stages on the same volume (device time of each call, median of 5):
+    # Sobel gradient source (+ normalisation), boundary seeds (stride 8, both forms), and the
+    # final dilation_conditional(fscn, label, 1, 50) of the step's labels
+    def stage_time(fn):
+        t = []
+        for _ in range(5):
+            fn()
+            t.append(ctx.kernel_timings()["stage_ms"])
+        return float(np.median(t))
+    grad = torch.empty(tuple(img.shape), dtype=torch.int32, device=dev)
+    sobel_k = stage_time(lambda: ctx.sobel(blurred, normalize=False, out=grad))
+    sobel_norm_ms = stage_time(lambda: ctx.sobel(blurred, normalize=True, out=grad))
+    seed_lab = torch.empty_like(lab0)
+    obj_seeds = seeds[torch.from_numpy(case.seed_labels > 0).to(dev)]
+    def seeds_call(geo):
+        seed_lab.copy_(lab0)
+        ctx.boundary_seeds(seed_lab, 8, geodesic=geo, seeds=obj_seeds)
+    seeds_uni_ms = stage_time(lambda: seeds_call(False))
+    seeds_geo_ms = stage_time(lambda: seeds_call(True))
+    dil_lab = torch.empty_like(lab)
+    def dil_call():
+        dil_lab.copy_(lab)
+        ctx.dilation_conditional(blurred, dil_lab, 50.0)
+    dil_ms = stage_time(dil_call)
     oift_k = float(np.mean(kern["oift_kernel_ms"]))
     relax_k = float(np.mean(kern["relax_loop_ms"])) / case.niter
     weights_k = float(np.mean(kern["weights_ms"]))
@@ -258,9 +283,14 @@ def main():
         "roofline": roof("k_oift_persistent", ALG["oift"], oift_k),
         "other_rooflines": [roof("k_relax2_march", ALG["relax_iter"], relax_k),
                             roof("k_weights", ALG["weights"], weights_k),
-                            roof("k_blur", ALG["blur2"], blur_k)],
+                            roof("k_blur", ALG["blur2"], blur_k),
+                            roof("k_sobel", ALG["sobel"], sobel_k)],
         "blur": {"passes": 2, "launches": 1, "ms": round(blur_k, 4),
                  "Mvoxels_per_s": round(n / (blur_k / 1e3) / 1e6, 1)},
+        "cli_stages_ms": {"sobel": round(sobel_k, 4), "sobel_normalized": round(sobel_norm_ms, 4),
+                          "boundary_seeds_stride8": round(seeds_uni_ms, 4),
+                          "boundary_seeds_geodesic8": round(seeds_geo_ms, 4),
+                          "dilation_conditional_p50": round(dil_ms, 4)},
     }
     if not args.no_batch:
         res["batched"] = batched_bench(ctx)

@@ -63,7 +63,8 @@ ROIFT_API int roift_last_timings(const roift_ctx *ctx, double *out6);
 /* Device-event times of the last call's main kernels, in milliseconds:
    [0] weight builder, [1] OIFT persistent kernel, [2] all relaxation iterations
    (back-to-back launches), [3] number of relaxation launches; with n >= 5,
-   [4] the last Gaussian blur (all passes).  n >= 4.
+   [4] the last Gaussian blur (all passes); with n >= 6, [5] the last Sobel /
+   distance / boundary-seed / dilation call (from its first kernel to its last).  n >= 4.
    (No reference counterpart: measurement only.) */
 ROIFT_API int roift_kernel_timings(const roift_ctx *ctx, double *out, int n);
 /* Wall-clock breakdown of the last OIFT's persistent kernel, measured on the
@@ -221,6 +222,40 @@ ROIFT_API int roift_relax_adjoint_dev(roift_ctx *ctx, const roift_problem *pb, c
  * (I16 or I32; the output has the input's dtype) and may be the same buffer.     */
 ROIFT_API int roift_gaussian_blur_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_in, int image_dtype,
                                       void *d_out, int passes);
+
+/* Gradient source of the experiments (exp_gradient_weight.cpp:593-607):
+ * gft::Scene32::SobelFilter (gft_filtering3.cpp:509-552) of the image into d_grad
+ * (N int32; the reference's linear range [1 + xs + xs*ys, N - 1 - xs - xs*ys) with
+ * its wrap-around neighbours, 0 elsewhere), bit-exact; with normalize != 0 then
+ * scaled as G = (int)((long long)G * max(image) / max(G)) when both are positive.  */
+ROIFT_API int roift_sobel_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                              int32_t *d_grad, int normalize);
+
+/* compute_bfs_edt (exp_geodesic_seeds.cpp:165-215): 6-connected BFS distance from
+ * the seeds (plain list) whose label is > 0, into d_dist (N int32; INT_MAX where
+ * unreached).  Computed exactly as the L1 distance by separable passes.           */
+ROIFT_API int roift_bfs_edt_dev(roift_ctx *ctx, const roift_problem *pb, const int32_t *d_label,
+                                const int32_t *d_seeds, int n_seeds, int32_t *d_dist);
+
+/* Background seeds on the volume faces, in the reference's raster order: with
+ * geodesic == 0 the CLI's inject_boundary_seeds (oiftrelax.cpp:175-222: voxels whose
+ * x, y and z are multiples of stride), with geodesic != 0 inject_boundary_seeds_geodesic
+ * (exp_geodesic_seeds.cpp:221-283, distances from d_seeds as roift_bfs_edt_dev).
+ * Unlabelled (NIL) candidates get label 0 in d_label and are written to d_new_seeds
+ * (append them to the seed list after the caller's seeds); *h_added = their count.
+ * ROIFT_EINVAL, with nothing modified, if more than `capacity` would be added.       */
+ROIFT_API int roift_boundary_seeds_dev(roift_ctx *ctx, const roift_problem *pb, int32_t *d_label, int stride,
+                                       int geodesic, const int32_t *d_seeds, int n_seeds, int32_t *d_new_seeds,
+                                       int capacity, int *h_added);
+
+/* The CLI's binary post-process dilation_conditional(fscn, label, radius, percentile)
+ * (oiftrelax.cpp:87-169, 403-409), radius = pb->adj_radius: label-0 voxels with a
+ * Spheric(radius) neighbour of label 1 form the border; stably sorted by image value,
+ * its first (int)(float)((percentile / 100.0) * (count - 1)) + 1 voxels get label 1.
+ * *h_set = that number.  An empty border changes nothing (the reference indexes out of
+ * range there).                                                                      */
+ROIFT_API int roift_dilation_conditional_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image,
+                                             int image_dtype, int32_t *d_label, float percentile, int *h_set);
 
 /* Host-pointer forms (copy in, run, copy out).  h_image is int32 as gft's Scene32. */
 ROIFT_API int roift_oift_fmax(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image,

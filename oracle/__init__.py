@@ -67,6 +67,15 @@ def lib():
         L.orc_estimate_sigma.argtypes = [i32p, C.c_int64]
         L.orc_gauss_kernel.argtypes = [f32p]
         L.orc_gaussian_blur.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_int, i32p]
+        L.orc_sobel.argtypes = [i32p, C.c_int, C.c_int, C.c_int, i32p]
+        L.orc_grad_normalize.argtypes = [i32p, C.c_int64, C.c_int32]
+        L.orc_bfs_edt.argtypes = [i32p, C.c_int, C.c_int, C.c_int, i32p, C.c_int, i32p]
+        L.orc_boundary_seeds.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_int, i32p]
+        L.orc_boundary_seeds.restype = C.c_int64
+        L.orc_boundary_seeds_geodesic.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_int, i32p, i32p]
+        L.orc_boundary_seeds_geodesic.restype = C.c_int64
+        L.orc_dilation_conditional.argtypes = [i32p, i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float]
+        L.orc_dilation_conditional.restype = C.c_int64
         L.orc_relax_diff_f64.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_float, i32p, i32p, C.c_int, C.c_int,
                                          f64p, C.c_double, C.c_double, f64p, C.c_double, f64p, f64p, f64p, f64p,
                                          f64p]
@@ -101,6 +110,11 @@ def ref_lib():
         L.ref_estimate_sigma.argtypes = [i32p, C.c_int, C.c_int, C.c_int]
         L.ref_gaussian_blur.argtypes = [i32p, C.c_int, C.c_int, C.c_int, C.c_int, i32p]
         L.ref_gauss_kernel.argtypes = [f32p, C.c_int]
+        L.ref_sobel.argtypes = [i32p, C.c_int, C.c_int, C.c_int, i32p]
+        L.ref_bfs_edt.argtypes = [i32p, C.c_int, C.c_int, C.c_int, i32p, i32p]
+        L.ref_boundary_seeds.argtypes = [i32p, C.c_int, C.c_int, C.c_int, i32p, C.c_int, i32p, C.c_int]
+        L.ref_boundary_seeds_geodesic.argtypes = [i32p, C.c_int, C.c_int, C.c_int, i32p, C.c_int, i32p, C.c_int]
+        L.ref_dilation_conditional.argtypes = [i32p, i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float]
         L.ref_estimate_sigma.restype = C.c_float
         _REF = L
     return _REF
@@ -324,6 +338,55 @@ def gaussian_blur(I, passes=1):
     return out
 
 
+def sobel(I):
+    """gft::Scene32::SobelFilter (orc_sobel)."""
+    I = _i32(I)
+    out = np.empty_like(I)
+    lib().orc_sobel(_p(I, C.c_int32), *_dims(I), _p(out, C.c_int32))
+    return out
+
+
+def grad_normalize(G, image_max):
+    """The experiments' gradient normalisation to [0, image_max] (orc_grad_normalize)."""
+    G = _i32(G).copy()
+    lib().orc_grad_normalize(_p(G, C.c_int32), G.size, int(image_max))
+    return G
+
+
+def bfs_edt(label, seeds):
+    """compute_bfs_edt: 6-connected BFS distance from the seeds whose label is > 0."""
+    label, S = _i32(label), _i32(seeds).ravel()
+    dist = np.empty_like(label)
+    lib().orc_bfs_edt(_p(label, C.c_int32), *_dims(label), _p(S, C.c_int32), S.size, _p(dist, C.c_int32))
+    return dist
+
+
+def boundary_seeds(label, stride):
+    """inject_boundary_seeds: returns (new label, appended seed indices)."""
+    label = _i32(label).copy()
+    out = np.empty(max(label.size, 1), np.int32)
+    k = lib().orc_boundary_seeds(_p(label, C.c_int32), *_dims(label), int(stride), _p(out, C.c_int32))
+    return label, out[:k].copy()
+
+
+def boundary_seeds_geodesic(label, stride, edt):
+    """inject_boundary_seeds_geodesic with the given BFS distances."""
+    label, edt = _i32(label).copy(), _i32(edt)
+    out = np.empty(max(label.size, 1), np.int32)
+    k = lib().orc_boundary_seeds_geodesic(_p(label, C.c_int32), *_dims(label), int(stride), _p(edt, C.c_int32),
+                                          _p(out, C.c_int32))
+    return label, out[:k].copy()
+
+
+def dilation_conditional(I, label, radius=1.0, percentile=50.0):
+    """The CLI's dilation_conditional: returns the new label map."""
+    I, label = _i32(I), _i32(label).copy()
+    n = lib().orc_dilation_conditional(_p(I, C.c_int32), _p(label, C.c_int32), *_dims(I), float(radius),
+                                       float(percentile))
+    assert n >= 0
+    return label
+
+
 # ---- the compiled reference (only where oracle/_ref was built) -------------------
 
 def ref_oift(I, seeds, label, adj_r=1.0, per=0.0, per_class=None):
@@ -415,3 +478,42 @@ def ref_gauss_kernel():
     v = np.zeros(8, np.float32)
     n = ref_lib().ref_gauss_kernel(_p(v, C.c_float), 8)
     return v[:n]
+
+
+def ref_sobel(I):
+    I = _i32(I)
+    out = np.empty_like(I)
+    ref_lib().ref_sobel(_p(I, C.c_int32), *_dims(I), _p(out, C.c_int32))
+    return out
+
+
+def ref_bfs_edt(label, seeds):
+    label, S = _i32(label), _gft_S(seeds)
+    dist = np.empty_like(label)
+    ref_lib().ref_bfs_edt(_p(label, C.c_int32), *_dims(label), _p(S, C.c_int32), _p(dist, C.c_int32))
+    return dist
+
+
+def _ref_inject(fn, label, seeds, stride):
+    label, S = _i32(label).copy(), _gft_S(seeds)
+    cap = S.size + label.size + 1
+    out = np.zeros(cap, np.int32)
+    added = fn(_p(label, C.c_int32), *_dims(label), _p(S, C.c_int32), int(stride), _p(out, C.c_int32), cap)
+    return label, out[1:1 + out[0]].copy(), added
+
+
+def ref_boundary_seeds(label, seeds, stride):
+    """The CLI's inject_boundary_seeds: (label, full seed list, count added)."""
+    return _ref_inject(ref_lib().ref_boundary_seeds, label, seeds, stride)
+
+
+def ref_boundary_seeds_geodesic(label, seeds, stride):
+    """BFS distances from the seeds, then inject_boundary_seeds_geodesic."""
+    return _ref_inject(ref_lib().ref_boundary_seeds_geodesic, label, seeds, stride)
+
+
+def ref_dilation_conditional(I, label, radius=1.0, percentile=50.0):
+    I, label = _i32(I), _i32(label).copy()
+    ref_lib().ref_dilation_conditional(_p(I, C.c_int32), _p(label, C.c_int32), *_dims(I), float(radius),
+                                       float(percentile))
+    return label

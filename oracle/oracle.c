@@ -980,3 +980,172 @@ int orc_gaussian_blur(const int32_t *I, int xs, int ys, int zs, int passes, int3
     free(tmp);
     return ORC_OK;
 }
+
+/* ------------------------------------------------------------------------ */
+/* gft::Scene32::SobelFilter (gft_filtering3.cpp:509-552): over the linear  */
+/* index range [di, N-di), di = 1 + xs + xs*ys (neighbours wrap across rows */
+/* and planes as the reference's address arithmetic does; every other      */
+/* voxel is 0), three integer 3x3 difference sums weighted (1 2 1; 2 4 2;   */
+/* 1 2 1), each shifted right by 4, then                                    */
+/*   ROUND(sqrtf((f0*f0 + f1*f1 + f2*f2) / 3.0))                            */
+/* with the FP32 sum promoted to double for the division and the quotient  */
+/* converted back to float for sqrtf.                                       */
+/* ------------------------------------------------------------------------ */
+int orc_sobel(const int32_t *I, int xs, int ys, int zs, int32_t *out)
+{
+    static const int n0[9] = {-1, 0, 1, -1, 0, 1, -1, 0, 1};
+    static const int n1[9] = {-1, -1, -1, 0, 0, 0, 1, 1, 1};
+    static const int we[9] = {1, 2, 1, 2, 4, 2, 1, 2, 1};
+    const int64_t N = (int64_t)xs * ys * zs, dx = 1, dy = xs, dz = (int64_t)xs * ys, di = dx + dy + dz;
+    memset(out, 0, sizeof(int32_t) * (size_t)N);
+    for (int64_t i = di; i < N - di; i++) {
+        int a0 = 0, a1 = 0, a2 = 0;
+        for (int m = 0; m < 9; m++) {
+            a0 += we[m] * (I[i + dz + dx * n0[m] + dy * n1[m]] - I[i - dz + dx * n0[m] + dy * n1[m]]);
+            a1 += we[m] * (I[i + dx + dz * n0[m] + dy * n1[m]] - I[i - dx + dz * n0[m] + dy * n1[m]]);
+            a2 += we[m] * (I[i + dy + dx * n0[m] + dz * n1[m]] - I[i - dy + dx * n0[m] + dz * n1[m]]);
+        }
+        a0 >>= 4;
+        a1 >>= 4;
+        a2 >>= 4;
+        const float f0 = (float)a0, f1 = (float)a1, f2 = (float)a2;
+        const float sum = f0 * f0 + f1 * f1 + f2 * f2;
+        const float g = sqrtf((float)((double)sum / 3.0));
+        out[i] = (g < 0) ? (int32_t)((double)g - 0.5) : (int32_t)((double)g + 0.5);
+    }
+    return ORC_OK;
+}
+
+/* The gradient normalisation of exp_gradient_weight.cpp:593-607: with Gmax = max G and
+   Wmax = max of the (blurred) image, G = (int)((long long)G * Wmax / Gmax) when both are
+   positive. */
+int orc_grad_normalize(int32_t *G, int64_t n, int32_t image_max)
+{
+    int32_t gmax = INT32_MIN;
+    for (int64_t p = 0; p < n; p++) if (G[p] > gmax) gmax = G[p];
+    if (gmax > 0 && image_max > 0)
+        for (int64_t p = 0; p < n; p++) G[p] = (int32_t)((long long)G[p] * image_max / gmax);
+    return ORC_OK;
+}
+
+/* compute_bfs_edt (exp_geodesic_seeds.cpp:165-215): 6-connected BFS from the seeds of S
+   whose label is > 0; unreached voxels keep INT32_MAX. */
+int orc_bfs_edt(const int32_t *label, int xs, int ys, int zs, const int32_t *S, int nS, int32_t *dist)
+{
+    const int64_t n = (int64_t)xs * ys * zs;
+    int64_t *q = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + nS + 1));
+    if (!q) return ORC_ENOMEM;
+    for (int64_t p = 0; p < n; p++) dist[p] = INT32_MAX;
+    int64_t head = 0, tail = 0;
+    for (int i = 0; i < nS; i++) {   /* a repeated seed is queued again, as in the reference */
+        const int64_t p = S[i];
+        if (p >= 0 && p < n && label[p] > 0) { dist[p] = 0; q[tail++] = p; }
+    }
+    static const int D[6][3] = {{0, 0, -1}, {0, -1, 0}, {-1, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    while (head < tail) {
+        const int64_t p = q[head++];
+        const int x = (int)(p % xs), y = (int)((p / xs) % ys), z = (int)(p / ((int64_t)xs * ys));
+        for (int i = 0; i < 6; i++) {
+            const int vx = x + D[i][0], vy = y + D[i][1], vz = z + D[i][2];
+            if (vx < 0 || vx >= xs || vy < 0 || vy >= ys || vz < 0 || vz >= zs) continue;
+            const int64_t r = vx + (int64_t)vy * xs + (int64_t)vz * xs * ys;
+            if (dist[r] == INT32_MAX) { dist[r] = dist[p] + 1; q[tail++] = r; }
+        }
+    }
+    free(q);
+    return ORC_OK;
+}
+
+/* inject_boundary_seeds (oiftrelax.cpp:175-222): on the grid x, y, z = 0, stride, 2 stride,
+   ... (raster order), every voxel on a volume face whose label is NIL becomes a background
+   seed (label 0) appended to `out`; returns the count. */
+int64_t orc_boundary_seeds(int32_t *label, int xs, int ys, int zs, int stride, int32_t *out)
+{
+    int64_t k = 0;
+    if (stride <= 0) return 0;
+    for (int z = 0; z < zs; z += stride)
+        for (int y = 0; y < ys; y += stride)
+            for (int x = 0; x < xs; x += stride) {
+                if (!(x == 0 || x == xs - 1 || y == 0 || y == ys - 1 || z == 0 || z == zs - 1)) continue;
+                const int64_t p = x + (int64_t)y * xs + (int64_t)z * xs * ys;
+                if (label[p] == -1) { label[p] = 0; out[k++] = (int32_t)p; }
+            }
+    return k;
+}
+
+/* inject_boundary_seeds_geodesic (exp_geodesic_seeds.cpp:221-283): every face voxel (raster
+   order) whose label is NIL and whose BFS distance d exceeds 2 stride becomes a background
+   seed when x, y or z is a multiple of the effective stride (stride for d <= 3 stride, else
+   2 stride). */
+int64_t orc_boundary_seeds_geodesic(int32_t *label, int xs, int ys, int zs, int stride, const int32_t *edt,
+                                    int32_t *out)
+{
+    int64_t k = 0;
+    const int threshold = stride * 2;
+    for (int z = 0; z < zs; z++)
+        for (int y = 0; y < ys; y++)
+            for (int x = 0; x < xs; x++) {
+                if (!(x == 0 || x == xs - 1 || y == 0 || y == ys - 1 || z == 0 || z == zs - 1)) continue;
+                const int64_t p = x + (int64_t)y * xs + (int64_t)z * xs * ys;
+                if (label[p] != -1) continue;
+                const int d = edt[p];
+                if (d <= threshold) continue;
+                int es;
+                if (d <= stride) es = stride / 2 > 1 ? stride / 2 : 1;
+                else if (d <= 3 * stride) es = stride;
+                else es = stride * 2;
+                if (x % es != 0 && y % es != 0 && z % es != 0) continue;
+                label[p] = 0;
+                out[k++] = (int32_t)p;
+            }
+    return k;
+}
+
+/* dilation_conditional (oiftrelax.cpp:87-169): the border = voxels of label 0 with a
+   Spheric(radius) neighbour of label 1; its voxels, stably sorted by image value (ties keep
+   raster order), are kept up to index (int)(float)((percentile / 100.0) * (count - 1)) and
+   set to label 1.  An empty border changes nothing (the reference indexes out of range
+   there).  Returns the number of voxels set. */
+typedef struct { int32_t v; int32_t p; } orc_vp;
+static int orc_vp_cmp(const void *a, const void *b)
+{
+    const orc_vp *x = (const orc_vp *)a, *y = (const orc_vp *)b;
+    if (x->v != y->v) return x->v < y->v ? -1 : 1;
+    return x->p < y->p ? -1 : (x->p > y->p);
+}
+
+int64_t orc_dilation_conditional(const int32_t *I, int32_t *label, int xs, int ys, int zs, float radius,
+                                 float percentile)
+{
+    orc_adj A;
+    if (adj_make(radius, &A) != ORC_OK) return -1;
+    const int64_t n = (int64_t)xs * ys * zs;
+    uint8_t *border = (uint8_t *)calloc((size_t)(n > 0 ? n : 1), 1);
+    if (!border) return -1;
+    for (int64_t p = 0; p < n; p++) {
+        if (label[p] != 1) continue;
+        const int x = (int)(p % xs), y = (int)((p / xs) % ys), z = (int)(p / ((int64_t)xs * ys));
+        for (int i = 0; i < A.n; i++) {
+            const int vx = x + A.dx[i], vy = y + A.dy[i], vz = z + A.dz[i];
+            if (vx < 0 || vx >= xs || vy < 0 || vy >= ys || vz < 0 || vz >= zs) continue;
+            const int64_t q = vx + (int64_t)vy * xs + (int64_t)vz * xs * ys;
+            if (label[q] == 0) border[q] = 1;
+        }
+    }
+    int64_t nb = 0;
+    for (int64_t p = 0; p < n; p++) nb += border[p];
+    int64_t set = 0;
+    if (nb > 0) {
+        orc_vp *v = (orc_vp *)malloc(sizeof(orc_vp) * (size_t)nb);
+        if (!v) { free(border); return -1; }
+        int64_t k = 0;
+        for (int64_t p = 0; p < n; p++) if (border[p]) { v[k].v = I[p]; v[k].p = (int32_t)p; k++; }
+        qsort(v, (size_t)nb, sizeof(orc_vp), orc_vp_cmp);
+        const float index_percentile = (float)((percentile / 100.0) * (double)(nb - 1));
+        const int64_t keep = (int64_t)(int)index_percentile + 1;
+        for (int64_t i = 0; i < nb && i < keep; i++) { label[v[i].p] = 1; set++; }
+        free(v);
+    }
+    free(border);
+    return set;
+}

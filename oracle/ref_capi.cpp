@@ -162,4 +162,15 @@ REF_API int ref_pipeline(const int *I, int xs, int ys, int zs, float adj_r, floa
     return 0;
 }
 
+// gft::Scene32::SobelFilter.
+REF_API int ref_sobel(const int *I, int xs, int ys, int zs, int *out)
+{
+    gft::sScene32 *scn = make_scene(I, xs, ys, zs);
+    gft::sScene32 *g = gft::Scene32::SobelFilter(scn);
+    std::memcpy(out, g->data, sizeof(int) * (size_t)g->n);
+    gft::Scene32::Destroy(&g);
+    gft::Scene32::Destroy(&scn);
+    return 0;
+}
+
 } // extern "C"

@@ -30,6 +30,7 @@ EXPORTS = (
     "roift_relax_binary_dev", "roift_oift_gradweight_dev", "roift_relax_gradweight_dev", "roift_relax_rbf_dev",
     "roift_relax_binary", "roift_batch_pipeline_dev", "roift_relax_adjoint_dev",
     "roift_gaussian_blur_dev", "roift_gaussian_blur",
+    "roift_sobel_dev", "roift_bfs_edt_dev", "roift_boundary_seeds_dev", "roift_dilation_conditional_dev",
 )
 
 
@@ -86,6 +87,10 @@ def lib():
                                               dp, vp]
         L.roift_gaussian_blur_dev.argtypes = [vp, pp, vp, ip, vp, ip]
         L.roift_gaussian_blur.argtypes = [vp, pp, vp, vp, ip]
+        L.roift_sobel_dev.argtypes = [vp, pp, vp, ip, vp, ip]
+        L.roift_bfs_edt_dev.argtypes = [vp, pp, vp, vp, ip, vp]
+        L.roift_boundary_seeds_dev.argtypes = [vp, pp, vp, ip, ip, vp, ip, vp, ip, C.POINTER(C.c_int)]
+        L.roift_dilation_conditional_dev.argtypes = [vp, pp, vp, ip, vp, fl, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -149,10 +154,10 @@ class Context:
 
     def kernel_timings(self):
         """Device-event ms of the last call's main kernels (weights, OIFT, relaxation loop, blur)."""
-        out = (C.c_double * 5)()
-        self._check(self._L.roift_kernel_timings(self._h, out, 5))
+        out = (C.c_double * 6)()
+        self._check(self._L.roift_kernel_timings(self._h, out, 6))
         return {"weights_ms": out[0], "oift_kernel_ms": out[1], "relax_loop_ms": out[2],
-                "relax_launches": int(out[3]), "blur_ms": out[4]}
+                "relax_launches": int(out[3]), "blur_ms": out[4], "stage_ms": out[5]}
 
     PARAMS = {"oift_blocks": 1, "flood_budget": 2, "relax_depth": 3, "flood_mode": 4, "relax_tile": 5,
               "blur_planes": 6}
@@ -270,6 +275,54 @@ class Context:
         pb, _ = make_problem(img.shape)
         self._check(self._L.roift_gaussian_blur(self._h, C.byref(pb), img.ctypes.data, out.ctypes.data, int(passes)))
         return out
+
+    def sobel(self, image, normalize=True, out=None):
+        """gft::Scene32::SobelFilter of an int16/int32 device volume into int32; with
+        normalize, scaled to [0, max(image)] as the gradient-weight experiment does."""
+        import torch
+        image = image.contiguous()
+        if out is None:
+            out = torch.empty(tuple(image.shape), dtype=torch.int32, device=image.device)
+        pb, _ = make_problem(image.shape)
+        self._check(self._L.roift_sobel_dev(self._h, C.byref(pb), image.data_ptr(), _dtype_code(image),
+                                            out.data_ptr(), int(bool(normalize))))
+        return out
+
+    def bfs_edt(self, label, seeds):
+        """6-connected BFS distance from the seeds whose label is > 0 (INT_MAX unreached)."""
+        import torch
+        seeds = seeds.to(device=label.device, dtype=torch.int32).contiguous()
+        out = torch.empty_like(label)
+        pb, _ = make_problem(label.shape)
+        self._check(self._L.roift_bfs_edt_dev(self._h, C.byref(pb), label.data_ptr(), seeds.data_ptr(),
+                                              int(seeds.numel()), out.data_ptr()))
+        return out
+
+    def boundary_seeds(self, label, stride, geodesic=False, seeds=None, capacity=None):
+        """Background seeds on the volume faces (label modified in place); returns the new
+        seed indices in raster order, to be appended after the caller's seeds."""
+        import torch
+        seeds = (torch.zeros(0, dtype=torch.int32, device=label.device) if seeds is None
+                 else seeds.to(device=label.device, dtype=torch.int32).contiguous())
+        zs, ys, xs = label.shape
+        if capacity is None:
+            capacity = 2 * (xs * ys + ys * zs + xs * zs) + 8
+        out = torch.empty(max(int(capacity), 1), dtype=torch.int32, device=label.device)
+        added = C.c_int(0)
+        pb, _ = make_problem(label.shape)
+        self._check(self._L.roift_boundary_seeds_dev(self._h, C.byref(pb), label.data_ptr(), int(stride),
+                                                     int(bool(geodesic)), seeds.data_ptr(), int(seeds.numel()),
+                                                     out.data_ptr(), int(capacity), C.byref(added)))
+        return out[: added.value]
+
+    def dilation_conditional(self, image, label, percentile=50.0, radius=1.0):
+        """The CLI's binary post-process, in place on `label`; returns the number of voxels set."""
+        pb, _ = make_problem(image.shape, radius)
+        n = C.c_int(0)
+        self._check(self._L.roift_dilation_conditional_dev(self._h, C.byref(pb), image.contiguous().data_ptr(),
+                                                           _dtype_code(image), label.data_ptr(), float(percentile),
+                                                           C.byref(n)))
+        return n.value
 
     def oift_gradweight(self, image, grad, seeds, label, alpha=0.5, adj_r=1.0, per=0.0, want_cost=True):
         """OIFT_Multi_GradWeight in place on `label`; grad: int32 device tensor; returns the cost map."""

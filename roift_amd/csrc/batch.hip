@@ -85,11 +85,9 @@ __global__ __launch_bounds__(256) void k_vol_extrema(const IT *__restrict__ img,
         lmax = max(lmax, val);
         lmin = min(lmin, val);
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        lmax = max(lmax, __shfl_xor(lmax, o));
-        lmin = min(lmin, __shfl_xor(lmin, o));
-    }
-    if (__lane_id() == 0) {
+    lmax = block_reduce(lmax, OpMax());
+    lmin = block_reduce(lmin, OpMin());
+    if (threadIdx.x == 0) {
         atomicMax(&ext[2 * blockIdx.y], lmax);
         atomicMin(&ext[2 * blockIdx.y + 1], lmin);
     }

@@ -194,6 +194,7 @@ ROIFT_API int roift_kernel_timings(const roift_ctx *c, double *out, int n)
     if (!c || !out || n < 4) return ROIFT_EINVAL;
     for (int i = 0; i < 4; i++) out[i] = c->ktimes[i];
     if (n >= 5) out[4] = c->ktimes[4];
+    if (n >= 6) out[5] = c->ktimes[5];
     return ROIFT_OK;
 }
 
@@ -536,6 +537,45 @@ ROIFT_API int roift_gaussian_blur(roift_ctx *ctx, const roift_problem *pb, const
         ROIFT_HIP_CHECK(hipMemcpyAsync(h_out, img, 4 * d.n, hipMemcpyDeviceToHost, ctx->stream));
         ROIFT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return ROIFT_OK;
+    });
+}
+
+ROIFT_API int roift_sobel_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                              int32_t *d_grad, int normalize)
+{
+    return guarded(ctx, [&]() -> int {
+        return run_sobel(*ctx, dims_of(pb), d_image, image_dtype, d_grad, normalize != 0);
+    });
+}
+
+ROIFT_API int roift_bfs_edt_dev(roift_ctx *ctx, const roift_problem *pb, const int32_t *d_label,
+                                const int32_t *d_seeds, int n_seeds, int32_t *d_dist)
+{
+    return guarded(ctx, [&]() -> int {
+        ev_record(*ctx, 14);
+        const int rc = run_bfs_edt(*ctx, dims_of(pb), d_label, d_seeds, n_seeds, d_dist);
+        ev_record(*ctx, 15);
+        ctx->ktimes[5] = ev_ms(*ctx, 14, 15);
+        return rc;
+    });
+}
+
+ROIFT_API int roift_boundary_seeds_dev(roift_ctx *ctx, const roift_problem *pb, int32_t *d_label, int stride,
+                                       int geodesic, const int32_t *d_seeds, int n_seeds, int32_t *d_new_seeds,
+                                       int capacity, int *h_added)
+{
+    return guarded(ctx, [&]() -> int {
+        return run_boundary_seeds(*ctx, dims_of(pb), d_label, stride, geodesic != 0, d_seeds, n_seeds, d_new_seeds,
+                                  capacity, h_added);
+    });
+}
+
+ROIFT_API int roift_dilation_conditional_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image,
+                                             int image_dtype, int32_t *d_label, float percentile, int *h_set)
+{
+    return guarded(ctx, [&]() -> int {
+        return run_dilation_conditional(*ctx, dims_of(pb), pb->adj_radius, d_image, image_dtype, d_label, percentile,
+                                        h_set);
     });
 }
 

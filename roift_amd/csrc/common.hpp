@@ -136,6 +136,35 @@ __device__ inline uint32_t wave_reserve(uint32_t *counter, uint32_t cnt)
     return base + incl - cnt;
 }
 
+// Workgroup reduction of one value per thread (1-D workgroups of at most 1024 threads, a
+// multiple of 64); the result is valid in thread 0.  Reductions publish one atomic per
+// workgroup: device-scope atomics on one address from many wavefronts serialise (several
+// ns each on this chip), so per-wavefront atomics on a shared counter cost far more than
+// the reduction itself.
+template <typename T, typename Op>
+__device__ inline T block_reduce(T v, Op op)
+{
+    __shared__ T part[16];
+    for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o));
+    const int w = (int)threadIdx.x / 64, nw = (int)blockDim.x / 64;
+    __syncthreads();   // part may still be read by a previous call
+    if (__lane_id() == 0) part[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 1; i < nw; i++) v = op(v, part[i]);
+    return v;
+}
+
+struct OpMax {
+    template <typename T> __device__ T operator()(T a, T b) const { return a > b ? a : b; }
+};
+struct OpMin {
+    template <typename T> __device__ T operator()(T a, T b) const { return a < b ? a : b; }
+};
+struct OpAdd {
+    template <typename T> __device__ T operator()(T a, T b) const { return a + b; }
+};
+
 // Fill without hipMemsetAsync (kernels are safe inside captured graphs).
 template <typename T>
 __global__ void k_fill(T *__restrict__ a, long long n, T v)

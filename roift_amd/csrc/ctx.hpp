@@ -26,8 +26,9 @@ struct roift_ctx {
     hipGraphExec_t relax_graph = nullptr;   // batched relaxation iterations, replayed while the batch is unchanged
     std::string relax_graph_key;
     // device-event times of the last call's main kernels, ms: weight builder, OIFT
-    // persistent kernel, all relaxation iterations; [3] = relaxation launches; [4] = Gaussian blur
-    double ktimes[5] = {0, 0, 0, 0, 0};
+    // persistent kernel, all relaxation iterations; [3] = relaxation launches; [4] = Gaussian blur;
+    // [5] = the last CLI stage call (Sobel, distance, boundary seeds or dilation), all its kernels
+    double ktimes[6] = {0, 0, 0, 0, 0, 0};
 };
 
 namespace roift {
@@ -137,6 +138,15 @@ struct BatchVolume {
 // iteration launches as a HIP graph.
 // gft::Scene32::GaussianBlur applied `passes` times (in place when d_in == d_out).
 int run_gaussian_blur(roift_ctx &c, const Dims &d, const void *d_in, int dtype, void *d_out, int passes);
+
+// The CLI's stages around the hot path (stages.hip).
+int run_sobel(roift_ctx &c, const Dims &d, const void *d_image, int dtype, int32_t *d_grad, bool normalize);
+int run_bfs_edt(roift_ctx &c, const Dims &d, const int32_t *d_label, const int32_t *d_seeds, int n_seeds,
+                int32_t *d_dist);
+int run_boundary_seeds(roift_ctx &c, const Dims &d, int32_t *d_label, int stride, bool geodesic,
+                       const int32_t *d_seeds, int n_seeds, int32_t *d_out, int capacity, int *h_added);
+int run_dilation_conditional(roift_ctx &c, const Dims &d, float radius, const void *d_image, int dtype,
+                             int32_t *d_label, float percentile, int *h_set);
 
 int run_relax_batch(roift_ctx &c, const std::vector<BatchVolume> &vols, const void *d_image, int image_dtype,
                     float per, int lab_obj, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter,

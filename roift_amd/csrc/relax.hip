@@ -171,14 +171,17 @@ __global__ __launch_bounds__(256) void k_relaxn(const IT *__restrict__ img, Rela
 __global__ void k_seed_labels(const int32_t *__restrict__ S, int ns, long long n, const int32_t *__restrict__ label,
                               int32_t *__restrict__ out, int *__restrict__ info)
 {
+    int m = INT32_MIN;
     for (long long i = gtid(); i < ns; i += gstride()) {
         const int v = S[i];
         int lb = NIL;
         if (v >= 0 && v < n) lb = label[v];
         else atomicOr(&info[1], 1);
         out[i] = lb;
-        atomicMax(&info[0], lb);
+        m = max(m, lb);
     }
+    m = block_reduce(m, OpMax());
+    if (threadIdx.x == 0) atomicMax(&info[0], m);
 }
 
 __global__ void k_mark_present(const int32_t *__restrict__ sl, int ns, int maxl, uint8_t *__restrict__ present)
@@ -489,8 +492,8 @@ __global__ void k_label_max(const int32_t *__restrict__ label, long long n, int 
 {
     int m = INT32_MIN;
     for (long long p = gtid(); p < n; p += gstride()) m = max(m, label[p]);
-    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
-    if (__lane_id() == 0) atomicMax(out, m);
+    m = block_reduce(m, OpMax());
+    if (threadIdx.x == 0) atomicMax(out, m);
 }
 
 __global__ void k_bin_init(const int32_t *__restrict__ label, long long n, float imax, float *__restrict__ f,
@@ -761,7 +764,7 @@ int run_relax_binary(roift_ctx &c, const Dims &d, int nadj, const void *d_image,
     int *info = buf<int>(c, "bin.info", 2);
     const int info0[2] = {INT32_MIN, 0};
     ROIFT_HIP_CHECK(hipMemcpyAsync(info, info0, sizeof(info0), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_label_max, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, d_label, n, info);
+    hipLaunchKernelGGL(k_label_max, dim3(grid_for(n, 256, 1024)), dim3(256), 0, st, d_label, n, info);
     ROIFT_HIP_CHECK(hipGetLastError());
     int hinfo[2];
     ROIFT_HIP_CHECK(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, st));

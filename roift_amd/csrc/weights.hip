@@ -85,14 +85,39 @@ __global__ __launch_bounds__(WX *WY) void k_weights(const IT *__restrict__ img, 
     const int tid = threadIdx.x + threadIdx.y * WX, wid = tid / 64;
     if (__lane_id() == 0) { smax[wid] = lmax; smin[wid] = lmin; }
     __syncthreads();
-    if (tid == 0) {
+    if (tid == 0) {   // this workgroup's own slots: no contended atomics (k_fold_maxmin folds them)
         for (int w = 1; w < WX * WY / 64; w++) { lmax = max(lmax, smax[w]); lmin = min(lmin, smin[w]); }
-        if (lmax != INT32_MIN) atomicMax(&ext[0], lmax);
-        if (lmin != INT32_MAX) atomicMin(&ext[1], lmin);
+        const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        ext[2 * b] = lmax;
+        ext[2 * b + 1] = lmin;
     }
     if (ovf) atomicOr(overflow, 1u);
 }
 
+// ext[0] = max of part[2b], ext[1] = min of part[2b + 1] over nb workgroup partials (one
+// workgroup of 1024 threads).
+__global__ __launch_bounds__(1024) void k_fold_maxmin(const int *__restrict__ part, int nb, int *__restrict__ ext)
+{
+    int lmax = INT32_MIN, lmin = INT32_MAX;
+    for (int i = threadIdx.x; i < nb; i += 1024) {
+        lmax = max(lmax, part[2 * i]);
+        lmin = min(lmin, part[2 * i + 1]);
+    }
+    __shared__ int smax[16], smin[16];
+    for (int o = 32; o > 0; o >>= 1) {
+        lmax = max(lmax, __shfl_xor(lmax, o));
+        lmin = min(lmin, __shfl_xor(lmin, o));
+    }
+    if (__lane_id() == 0) { smax[threadIdx.x / 64] = lmax; smin[threadIdx.x / 64] = lmin; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 16; w++) { lmax = max(lmax, smax[w]); lmin = min(lmin, smin[w]); }
+        ext[0] = lmax;
+        ext[1] = lmin;
+    }
+}
+
+// per-workgroup extrema into ext[2b], ext[2b + 1] (folded by k_fold_maxmin)
 template <typename IT>
 __global__ __launch_bounds__(256) void k_extrema(const IT *__restrict__ img, long long n, int *__restrict__ ext)
 {
@@ -112,8 +137,8 @@ __global__ __launch_bounds__(256) void k_extrema(const IT *__restrict__ img, lon
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < (int)(blockDim.x / 64); w++) { lmax = max(lmax, smax[w]); lmin = min(lmin, smin[w]); }
-        atomicMax(&ext[0], lmax);
-        atomicMin(&ext[1], lmin);
+        ext[2 * blockIdx.x] = lmax;
+        ext[2 * blockIdx.x + 1] = lmin;
     }
 }
 
@@ -128,11 +153,9 @@ __global__ __launch_bounds__(256) void k_moments(const IT *__restrict__ img, lon
         s1 += v;
         s2 += v * v;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        s1 += __shfl_xor(s1, o);
-        s2 += __shfl_xor(s2, o);
-    }
-    if (__lane_id() == 0) {
+    s1 = block_reduce(s1, OpAdd());
+    s2 = block_reduce(s2, OpAdd());
+    if (threadIdx.x == 0) {
         atomicAdd(&out[0], (unsigned long long)s1);
         atomicAdd(&out[1], (unsigned long long)s2);
     }
@@ -140,13 +163,18 @@ __global__ __launch_bounds__(256) void k_moments(const IT *__restrict__ img, lon
 
 namespace {
 
+dim3 weights_grid(const Dims &d)
+{
+    return dim3((d.xs + WX - 1) / WX, (d.ys + WY - 1) / WY, (d.zs + WZ - 1) / WZ);
+}
+
 template <int NADJ, typename IT, typename WT>
 void launch_weights(roift_ctx &c, const Dims &d, const void *img, double fp, double fm, const GradArgs &ga, void *wa,
                     void *wb, int *ext, uint32_t *ovf)
 {
     ROIFT_REQUIRE((d.ys + WY - 1) / WY <= 65535 && (d.zs + WZ - 1) / WZ <= 65535, ROIFT_EUNSUPPORTED,
                   "volume too large for the weight builder's grid");
-    const dim3 grid((d.xs + WX - 1) / WX, (d.ys + WY - 1) / WY, (d.zs + WZ - 1) / WZ);
+    const dim3 grid = weights_grid(d);
     if (ga.G)
         hipLaunchKernelGGL((k_weights<NADJ, IT, WT, true>), grid, dim3(WX, WY), 0, c.stream, (const IT *)img, d, fp,
                            fm, ga, (WT *)wa, (WT *)wb, ext, ovf);
@@ -168,6 +196,9 @@ WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_im
     uint32_t *ovf = reinterpret_cast<uint32_t *>(ext + 2);
     const int init[4] = {INT32_MIN, INT32_MAX, 0, 0};
     ROIFT_HIP_CHECK(hipMemcpyAsync(ext, init, sizeof(init), hipMemcpyHostToDevice, c.stream));
+    const dim3 wg = weights_grid(d);
+    const int nblk = (int)(wg.x * wg.y * wg.z);
+    int *part = buf<int>(c, "w.part", 2 * (size_t)nblk);
     // per_pq / 100.0 in double, as the reference's "w *= (1.0 +- per_pq/100.0)"
     const double fp = 1.0 + (double)per / 100.0, fm = 1.0 - (double)per / 100.0;
     GradArgs ga{nullptr, 0.0f, 0.0f, 0.0f};
@@ -180,16 +211,18 @@ WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_im
     const bool i16 = image_dtype == ROIFT_I16, w16 = weight_dtype == ROIFT_U16;
     ev_record(c, 8);
     if (nadj == 7) {
-        if (i16 && w16) launch_weights<7, int16_t, uint16_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
-        else if (i16) launch_weights<7, int16_t, uint32_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
-        else if (w16) launch_weights<7, int32_t, uint16_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
-        else launch_weights<7, int32_t, uint32_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
+        if (i16 && w16) launch_weights<7, int16_t, uint16_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, part, ovf);
+        else if (i16) launch_weights<7, int16_t, uint32_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, part, ovf);
+        else if (w16) launch_weights<7, int32_t, uint16_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, part, ovf);
+        else launch_weights<7, int32_t, uint32_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, part, ovf);
     } else {
-        if (i16 && w16) launch_weights<27, int16_t, uint16_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
-        else if (i16) launch_weights<27, int16_t, uint32_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
-        else if (w16) launch_weights<27, int32_t, uint16_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
-        else launch_weights<27, int32_t, uint32_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, ext, ovf);
+        if (i16 && w16) launch_weights<27, int16_t, uint16_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, part, ovf);
+        else if (i16) launch_weights<27, int16_t, uint32_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, part, ovf);
+        else if (w16) launch_weights<27, int32_t, uint16_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, part, ovf);
+        else launch_weights<27, int32_t, uint32_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, part, ovf);
     }
+    hipLaunchKernelGGL(k_fold_maxmin, dim3(1), dim3(1024), 0, c.stream, (const int *)part, nblk, ext);
+    ROIFT_HIP_CHECK(hipGetLastError());
     ev_record(c, 9);
     int host[4];
     ROIFT_HIP_CHECK(hipMemcpyAsync(host, ext, sizeof(host), hipMemcpyDeviceToHost, c.stream));
@@ -204,12 +237,14 @@ WeightInfo image_extrema(roift_ctx &c, const Dims &d, const void *d_image, int i
     int *ext = buf<int>(c, "w.ext", 4);
     const int init[4] = {INT32_MIN, INT32_MAX, 0, 0};
     ROIFT_HIP_CHECK(hipMemcpyAsync(ext, init, sizeof(init), hipMemcpyHostToDevice, c.stream));
+    const unsigned g = grid_for(d.n, 256, 4096);
+    int *part = buf<int>(c, "w.part", 2 * (size_t)g);
     if (image_dtype == ROIFT_I16)
-        hipLaunchKernelGGL(k_extrema<int16_t>, dim3(grid_for(d.n, 256, 4096)), dim3(256), 0, c.stream,
-                           (const int16_t *)d_image, d.n, ext);
+        hipLaunchKernelGGL(k_extrema<int16_t>, dim3(g), dim3(256), 0, c.stream, (const int16_t *)d_image, d.n, part);
     else
-        hipLaunchKernelGGL(k_extrema<int32_t>, dim3(grid_for(d.n, 256, 4096)), dim3(256), 0, c.stream,
-                           (const int32_t *)d_image, d.n, ext);
+        hipLaunchKernelGGL(k_extrema<int32_t>, dim3(g), dim3(256), 0, c.stream, (const int32_t *)d_image, d.n, part);
+    ROIFT_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_fold_maxmin, dim3(1), dim3(1024), 0, c.stream, (const int *)part, (int)g, ext);
     ROIFT_HIP_CHECK(hipGetLastError());
     int host[4];
     ROIFT_HIP_CHECK(hipMemcpyAsync(host, ext, sizeof(host), hipMemcpyDeviceToHost, c.stream));
@@ -227,10 +262,10 @@ float image_sigma(roift_ctx &c, const Dims &d, const void *d_image, int image_dt
     const unsigned long long zero[2] = {0, 0};
     ROIFT_HIP_CHECK(hipMemcpyAsync(m, zero, sizeof(zero), hipMemcpyHostToDevice, c.stream));
     if (image_dtype == ROIFT_I16)
-        hipLaunchKernelGGL(k_moments<int16_t>, dim3(grid_for(d.n, 256, 4096)), dim3(256), 0, c.stream,
+        hipLaunchKernelGGL(k_moments<int16_t>, dim3(grid_for(d.n, 256, 1024)), dim3(256), 0, c.stream,
                            (const int16_t *)d_image, d.n, m);
     else
-        hipLaunchKernelGGL(k_moments<int32_t>, dim3(grid_for(d.n, 256, 4096)), dim3(256), 0, c.stream,
+        hipLaunchKernelGGL(k_moments<int32_t>, dim3(grid_for(d.n, 256, 1024)), dim3(256), 0, c.stream,
                            (const int32_t *)d_image, d.n, m);
     ROIFT_HIP_CHECK(hipGetLastError());
     unsigned long long h[2];

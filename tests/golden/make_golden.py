@@ -3,14 +3,17 @@
 Runs gft::ift::OIFT_Multi[_PerClass] and ORelax_1_Multi[_PerClass] of the
 reference, and (into variants/) its ORelax_1 binary forms and the experiment
 variants OIFT_Multi_GradWeight / ORelax_1_Multi_GradWeight / ORelax_1_Multi_GaussRBF, and (into
-blur/) gft::Scene32::GaussianBlur applied 1-3 times (oracle/_ref/libroift_ref.so, built by
-`make -C oracle ref` from the reference sources) on small deterministic inputs and stores inputs and
+blur/) gft::Scene32::GaussianBlur applied 1-3 times, and (into stages/) SobelFilter,
+compute_bfs_edt, the CLI's and the geodesic experiment's boundary seeds and the CLI's
+dilation_conditional (oracle/_ref/libroift_ref.so, built by `make -C oracle ref` from the
+reference sources) on small deterministic inputs and stores inputs and
 outputs as .npz.  The reference holds no fixtures of its own, so these are
 the reference-produced known answers the oracle and the GPU path are pinned
 to.  Re-run only if the inputs below change:
 
     python tests/golden/make_golden.py          # everything
     python tests/golden/make_golden.py blur     # only blur/
+    python tests/golden/make_golden.py stages   # only stages/
 """
 import os
 import sys
@@ -155,11 +158,57 @@ def write_blur():
         print("blur", name, img.shape)
 
 
+def write_stages():
+    out_dir = os.path.join(HERE, "stages")
+    os.makedirs(out_dir, exist_ok=True)
+    rng = np.random.default_rng(78)
+    # Sobel: random, a blurred phantom, signed values (negative sums shift arithmetically)
+    sob = {"rand": rng.integers(0, 4000, size=(9, 10, 11)).astype(np.int32),
+           "phantom": O.ref_gaussian_blur(P.small_case((12, 14, 16), 11, per=50.0).image, 2),
+           "signed": rng.integers(-20000, 20000, size=(6, 7, 8)).astype(np.int32),
+           "plane": rng.integers(0, 255, size=(1, 12, 13)).astype(np.int32)}
+    for name, img in sob.items():
+        np.savez_compressed(os.path.join(out_dir, f"sobel_{name}.npz"), image=img, grad=O.ref_sobel(img))
+        print("sobel", name, img.shape)
+    # seeds: object seeds (label 1), background seeds (label 0) and an out-of-range entry
+    for name, shape in (("vol", (14, 17, 19)), ("flat", (1, 20, 30)), ("thin", (3, 2, 40))):
+        n = int(np.prod(shape))
+        lab = np.full(n, -1, np.int32)
+        obj = rng.choice(n, size=3, replace=False)
+        bg = rng.choice(np.setdiff1d(np.arange(n), obj), size=4, replace=False)
+        lab[obj] = 1
+        lab[bg] = 0
+        lab = lab.reshape(shape)
+        seeds = np.concatenate([obj, bg, [n + 3]]).astype(np.int32)
+        rec = dict(label=lab, seeds=seeds, edt=O.ref_bfs_edt(lab, seeds))
+        for stride in (1, 2, 3, 5, 8):
+            l1, s1, _ = O.ref_boundary_seeds(lab, seeds, stride)
+            l2, s2, _ = O.ref_boundary_seeds_geodesic(lab, seeds, stride)
+            rec[f"uni{stride}_label"], rec[f"uni{stride}_seeds"] = l1, s1
+            rec[f"geo{stride}_label"], rec[f"geo{stride}_seeds"] = l2, s2
+        np.savez_compressed(os.path.join(out_dir, f"seeds_{name}.npz"), **rec)
+        print("seeds", name, shape)
+    # conditional dilation of an OIFT result; the quantised image has many ties
+    c = P.small_case((16, 18, 20), 12, per=50.0)
+    img = O.ref_gaussian_blur(c.image, 2)
+    lab = (O.ref_oift(img, c.seeds, c.label_init(), 1.0, c.per) > 0).astype(np.int32)
+    for name, im in (("smooth", img), ("ties", (img // 64) * 64)):
+        rec = dict(image=im, label=lab)
+        for pc in (0.0, 25.0, 50.0, 90.0, 100.0):
+            for r in (1.0, 1.5):
+                rec[f"p{int(pc)}_r{r}"] = O.ref_dilation_conditional(im, lab, r, pc)
+        np.savez_compressed(os.path.join(out_dir, f"dilation_{name}.npz"), **rec)
+        print("dilation", name)
+
+
 def main():
     if O.ref_lib() is None:
         raise SystemExit("oracle/_ref/libroift_ref.so is not built (make -C oracle ref)")
     if sys.argv[1:] == ["blur"]:
         write_blur()
+        return
+    if sys.argv[1:] == ["stages"]:
+        write_stages()
         return
     for name, c in cases():
         img = np.ascontiguousarray(c["image"], np.int32)
@@ -177,6 +226,7 @@ def main():
         print(name, img.shape, "obj voxels", int((oift > 0).sum()), int((relax > 0).sum()))
     write_variants()
     write_blur()
+    write_stages()
 
 
 if __name__ == "__main__":
