@@ -35,7 +35,7 @@ enum roift_status {
     ROIFT_EUNSUPPORTED = -4 /* valid input outside what this build implements (see roift_last_error) */
 };
 
-enum roift_dtype { ROIFT_I16 = 1, ROIFT_I32 = 2, ROIFT_U16 = 3, ROIFT_U32 = 4, ROIFT_F32 = 5 };
+enum roift_dtype { ROIFT_I16 = 1, ROIFT_I32 = 2, ROIFT_U16 = 3, ROIFT_U32 = 4, ROIFT_F32 = 5, ROIFT_U8 = 6, ROIFT_I8 = 7 };
 
 typedef struct roift_ctx roift_ctx;
 
@@ -256,6 +256,47 @@ ROIFT_API int roift_boundary_seeds_dev(roift_ctx *ctx, const roift_problem *pb, 
  * range there).                                                                      */
 ROIFT_API int roift_dilation_conditional_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image,
                                              int image_dtype, int32_t *d_label, float percentile, int *h_set);
+
+/* ---- volume, seed and polarity files (the CLI's argv contract) ------------------- */
+/* Formats: SCN (gft's raw scene, 8/16/32-bit; gft_scene32.cpp:334-419, 602-664) and
+ * NIfTI-1 (.nii, .nii.gz, .hdr/.img; uint8, int8, int16, uint16, int32 and float32 as
+ * gft::Scene32::ReadNifti1 accepts them, gft_scene32.cpp:525-597; float32 becomes
+ * ROUND(v * (10000 / max)) as there).  Host-only entries report errors through
+ * roift_io_last_error().                                                            */
+enum roift_volume_format { ROIFT_FMT_SCN = 1, ROIFT_FMT_NIFTI = 2 };
+typedef struct {
+    int xsize, ysize, zsize;
+    float dx, dy, dz;          /* voxel spacing */
+    int stored_dtype;          /* ROIFT_U8 / I8 / U16 / I16 / I32 / F32, as stored */
+    int format;                /* roift_volume_format */
+} roift_volume_info;
+enum { ROIFT_READ_SHIFT_MIN = 1 };   /* add -min when the minimum is negative (oiftrelax.cpp:263-268) */
+
+ROIFT_API const char *roift_io_last_error(void);
+/* Header only. */
+ROIFT_API int roift_volume_info_read(const char *path, roift_volume_info *info);
+/* gft::Scene32::Read (gft_scene32.cpp:334-419): the N samples as int32. */
+ROIFT_API int roift_volume_read(const char *path, int32_t *h_data, long long n);
+/* gft::Scene32::Write of a label volume (gft_scene32.cpp:602-664): .scn (8/16/32 bits by
+ * the maximum) or int32 NIfTI-1 (.nii / .nii.gz); like_path (nullable) is the input
+ * volume, whose NIfTI header fields are kept as gft keeps the cloned header.          */
+ROIFT_API int roift_volume_write(const char *path, const int32_t *h_data, const roift_volume_info *info,
+                                 const char *like_path);
+/* The CLI's seeds file (oiftrelax.cpp:323-351): "<n>" then n lines "x y z id label";
+ * voxels outside the volume are skipped, negative labels become 0 (*h_clamped = 1).
+ * h_seeds / h_labels receive up to capacity entries in file order; *h_count = all
+ * (with both null, it only counts).                                                 */
+ROIFT_API int roift_seeds_read(const char *path, int xsize, int ysize, int zsize, int32_t *h_seeds,
+                               int32_t *h_labels, int capacity, int *h_count, int *h_clamped);
+/* The CLI's per-class polarity file (oiftrelax.cpp:271-310): "<n>" then n values in
+ * [-1, 1]; h_per_class receives them times 100.0f.                                   */
+ROIFT_API int roift_polarity_read(const char *path, float *h_per_class, int capacity, int *h_count);
+/* Reads a volume straight into device memory (d_dst: N values of dst_dtype, I16 or I32):
+ * the raw samples go through two pinned host buffers, double-buffered against the file
+ * reads, and are converted on the device.  With ROIFT_READ_SHIFT_MIN a negative minimum
+ * is added back (*h_shift = the amount).  ROIFT_EINVAL when the values do not fit I16.  */
+ROIFT_API int roift_volume_read_dev(roift_ctx *ctx, const char *path, void *d_dst, int dst_dtype, int flags,
+                                    int *h_shift, roift_volume_info *info);
 
 /* Host-pointer forms (copy in, run, copy out).  h_image is int32 as gft's Scene32. */
 ROIFT_API int roift_oift_fmax(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image,

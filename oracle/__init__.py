@@ -115,6 +115,10 @@ def ref_lib():
         L.ref_boundary_seeds.argtypes = [i32p, C.c_int, C.c_int, C.c_int, i32p, C.c_int, i32p, C.c_int]
         L.ref_boundary_seeds_geodesic.argtypes = [i32p, C.c_int, C.c_int, C.c_int, i32p, C.c_int, i32p, C.c_int]
         L.ref_dilation_conditional.argtypes = [i32p, i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float]
+        L.ref_cli_main.argtypes = [C.c_int, C.POINTER(C.c_char_p)]
+        L.ref_scene_dims.argtypes = [C.c_char_p, C.POINTER(C.c_int)]
+        L.ref_scene_read.argtypes = [C.c_char_p, i32p, C.c_longlong]
+        L.ref_scene_write.argtypes = [C.c_char_p, i32p, C.c_int, C.c_int, C.c_int]
         L.ref_estimate_sigma.restype = C.c_float
         _REF = L
     return _REF
@@ -517,3 +521,24 @@ def ref_dilation_conditional(I, label, radius=1.0, percentile=50.0):
     ref_lib().ref_dilation_conditional(_p(I, C.c_int32), _p(label, C.c_int32), *_dims(I), float(radius),
                                        float(percentile))
     return label
+
+
+def ref_cli(args):
+    """Runs the reference CLI (oiftrelax.cpp's main) in this process with argv = args."""
+    argv = ["oiftrelax"] + [str(a) for a in args]
+    arr = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
+    return ref_lib().ref_cli_main(len(argv), arr)
+
+
+def ref_scene_read(path):
+    """gft::Scene32::Read -> int32 array (zs, ys, xs)."""
+    d = (C.c_int * 3)()
+    ref_lib().ref_scene_dims(str(path).encode(), d)
+    out = np.empty((d[2], d[1], d[0]), np.int32)
+    assert ref_lib().ref_scene_read(str(path).encode(), _p(out, C.c_int32), out.size) == 0
+    return out
+
+
+def ref_scene_write(path, vol):
+    vol = _i32(vol)
+    ref_lib().ref_scene_write(str(path).encode(), _p(vol, C.c_int32), *_dims(vol))

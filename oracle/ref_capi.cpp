@@ -173,4 +173,35 @@ REF_API int ref_sobel(const int *I, int xs, int ys, int zs, int *out)
     return 0;
 }
 
+// gft::Scene32::Read / Write on files.
+REF_API int ref_scene_dims(const char *path, int *dims)
+{
+    gft::sScene32 *s = gft::Scene32::Read((char *)path);
+    dims[0] = s->xsize;
+    dims[1] = s->ysize;
+    dims[2] = s->zsize;
+    gft::Scene32::Destroy(&s);
+    return 0;
+}
+
+REF_API int ref_scene_read(const char *path, int *out, long long n)
+{
+    gft::sScene32 *s = gft::Scene32::Read((char *)path);
+    if ((long long)s->n != n) {
+        gft::Scene32::Destroy(&s);
+        return -1;
+    }
+    std::memcpy(out, s->data, sizeof(int) * (size_t)n);
+    gft::Scene32::Destroy(&s);
+    return 0;
+}
+
+REF_API int ref_scene_write(const char *path, const int *data, int xs, int ys, int zs)
+{
+    gft::sScene32 *s = make_scene(data, xs, ys, zs);
+    gft::Scene32::Write(s, (char *)path);
+    gft::Scene32::Destroy(&s);
+    return 0;
+}
+
 } // extern "C"

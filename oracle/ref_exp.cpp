@@ -171,4 +171,7 @@ REF_API int ref_dilation_conditional(const int *I, int *label, int xs, int ys, i
     return 0;
 }
 
+// the reference CLI itself (oiftrelax.cpp:227-432), run in-process on files
+REF_API int ref_cli_main(int argc, char **argv) { return cli::main(argc, argv); }
+
 }  // extern "C"

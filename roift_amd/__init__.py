@@ -31,6 +31,8 @@ EXPORTS = (
     "roift_relax_binary", "roift_batch_pipeline_dev", "roift_relax_adjoint_dev",
     "roift_gaussian_blur_dev", "roift_gaussian_blur",
     "roift_sobel_dev", "roift_bfs_edt_dev", "roift_boundary_seeds_dev", "roift_dilation_conditional_dev",
+    "roift_io_last_error", "roift_volume_info_read", "roift_volume_read", "roift_volume_write", "roift_seeds_read",
+    "roift_polarity_read", "roift_volume_read_dev",
 )
 
 
@@ -38,6 +40,16 @@ class RoiftError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"roift error {code}: {msg}")
         self.code = code
+
+
+class VolumeInfo(C.Structure):
+    _fields_ = [("xsize", C.c_int), ("ysize", C.c_int), ("zsize", C.c_int), ("dx", C.c_float), ("dy", C.c_float),
+                ("dz", C.c_float), ("stored_dtype", C.c_int), ("format", C.c_int)]
+
+
+U8, I8 = 6, 7
+FMT_SCN, FMT_NIFTI = 1, 2
+READ_SHIFT_MIN = 1
 
 
 class Problem(C.Structure):
@@ -55,6 +67,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with make -C roift_amd/csrc")
+        # torch bundles its own HIP runtime (soname libamdhip64.so.7, as ROCm's).  Loaded
+        # first, it is the one libroift.so binds to; loaded after libroift.so, it would
+        # come in as a second runtime beside ROCm's, and the two do not share devices.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp, ip, i32p, i8p, dp = C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int8), C.POINTER(C.c_double)
         pp = C.POINTER(Problem)
@@ -87,6 +106,15 @@ def lib():
                                               dp, vp]
         L.roift_gaussian_blur_dev.argtypes = [vp, pp, vp, ip, vp, ip]
         L.roift_gaussian_blur.argtypes = [vp, pp, vp, vp, ip]
+        L.roift_io_last_error.argtypes = []
+        L.roift_io_last_error.restype = C.c_char_p
+        vip = C.POINTER(VolumeInfo)
+        L.roift_volume_info_read.argtypes = [C.c_char_p, vip]
+        L.roift_volume_read.argtypes = [C.c_char_p, vp, C.c_longlong]
+        L.roift_volume_write.argtypes = [C.c_char_p, vp, vip, C.c_char_p]
+        L.roift_seeds_read.argtypes = [C.c_char_p, ip, ip, ip, vp, vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.roift_polarity_read.argtypes = [C.c_char_p, vp, ip, C.POINTER(C.c_int)]
+        L.roift_volume_read_dev.argtypes = [vp, C.c_char_p, vp, ip, ip, C.POINTER(C.c_int), vip]
         L.roift_sobel_dev.argtypes = [vp, pp, vp, ip, vp, ip]
         L.roift_bfs_edt_dev.argtypes = [vp, pp, vp, vp, ip, vp]
         L.roift_boundary_seeds_dev.argtypes = [vp, pp, vp, ip, ip, vp, ip, vp, ip, C.POINTER(C.c_int)]
@@ -98,6 +126,55 @@ def lib():
 def exported_symbols():
     L = lib()
     return {name: hasattr(L, name) for name in EXPORTS}
+
+
+def _io_check(rc):
+    if rc != OK:
+        raise RoiftError(rc, lib().roift_io_last_error().decode())
+
+
+def volume_info(path):
+    """Header of a .scn / .nii / .nii.gz / .hdr volume."""
+    info = VolumeInfo()
+    _io_check(lib().roift_volume_info_read(str(path).encode(), C.byref(info)))
+    return info
+
+
+def read_volume(path):
+    """gft::Scene32::Read: int32 array of shape (zs, ys, xs) and the header."""
+    info = volume_info(path)
+    out = np.empty((info.zsize, info.ysize, info.xsize), np.int32)
+    _io_check(lib().roift_volume_read(str(path).encode(), out.ctypes.data, out.size))
+    return out, info
+
+
+def write_volume(path, vol, spacing=(1.0, 1.0, 1.0), like=None):
+    """gft::Scene32::Write of an int32 label volume (.scn, .nii, .nii.gz)."""
+    vol = np.ascontiguousarray(vol, np.int32)
+    zs, ys, xs = vol.shape
+    info = VolumeInfo(xs, ys, zs, spacing[0], spacing[1], spacing[2], I32, 0)
+    _io_check(lib().roift_volume_write(str(path).encode(), vol.ctypes.data, C.byref(info),
+                                       None if like is None else str(like).encode()))
+
+
+def read_seeds(path, shape):
+    """The CLI's seeds file: (voxel indices, labels) in file order."""
+    zs, ys, xs = shape
+    n, clamped = C.c_int(0), C.c_int(0)
+    _io_check(lib().roift_seeds_read(str(path).encode(), xs, ys, zs, None, None, 0, C.byref(n), C.byref(clamped)))
+    seeds = np.empty(max(n.value, 1), np.int32)
+    labels = np.empty_like(seeds)
+    _io_check(lib().roift_seeds_read(str(path).encode(), xs, ys, zs, seeds.ctypes.data, labels.ctypes.data, n.value,
+                                     C.byref(n), C.byref(clamped)))
+    return seeds[: n.value], labels[: n.value], bool(clamped.value)
+
+
+def read_polarity(path):
+    """The CLI's per-class polarity file, times 100."""
+    buf = np.empty(4096, np.float32)
+    n = C.c_int(0)
+    _io_check(lib().roift_polarity_read(str(path).encode(), buf.ctypes.data, buf.size, C.byref(n)))
+    return buf[: n.value].copy()
 
 
 def make_problem(shape, adj_r=1.0, per=0.0, per_class=None):
@@ -268,6 +345,19 @@ class Context:
         self._check(self._L.roift_gaussian_blur_dev(self._h, C.byref(pb), image.data_ptr(), _dtype_code(image),
                                                     out.data_ptr(), int(passes)))
         return out
+
+    def read_volume_dev(self, path, dtype=None, shift_min=False):
+        """Reads a volume into a new device tensor (int16 or int32) through pinned staging;
+        returns (tensor, info, shift)."""
+        import torch
+        info = volume_info(path)
+        dt = torch.int16 if dtype in (None, torch.int16) else torch.int32
+        out = torch.empty((info.zsize, info.ysize, info.xsize), dtype=dt, device=f"cuda:{self.device}")
+        shift = C.c_int(0)
+        self._check(self._L.roift_volume_read_dev(self._h, str(path).encode(), out.data_ptr(), _dtype_code(out),
+                                                  READ_SHIFT_MIN if shift_min else 0, C.byref(shift),
+                                                  C.byref(info)))
+        return out, info, shift.value
 
     def gaussian_blur_host(self, image, passes=2):
         img = np.ascontiguousarray(image, np.int32)

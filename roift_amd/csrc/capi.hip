@@ -166,6 +166,10 @@ ROIFT_API void roift_destroy(roift_ctx *c)
         if (e) (void)hipEventDestroy(e);
     if (c->h_ctr) (void)hipHostFree(c->h_ctr);
     if (c->relax_graph) (void)hipGraphExecDestroy(c->relax_graph);
+    for (auto &p : c->pinned)
+        if (p) (void)hipHostFree(p);
+    for (auto &e : c->io_ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->owns_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -577,6 +581,36 @@ ROIFT_API int roift_dilation_conditional_dev(roift_ctx *ctx, const roift_problem
         return run_dilation_conditional(*ctx, dims_of(pb), pb->adj_radius, d_image, image_dtype, d_label, percentile,
                                         h_set);
     });
+}
+
+ROIFT_API const char *roift_io_last_error(void) { return io_last_error(); }
+
+ROIFT_API int roift_volume_info_read(const char *path, roift_volume_info *info) { return volume_info(path, info); }
+
+ROIFT_API int roift_volume_read(const char *path, int32_t *h_data, long long n) { return volume_read(path, h_data, n); }
+
+ROIFT_API int roift_volume_write(const char *path, const int32_t *h_data, const roift_volume_info *info,
+                                 const char *like_path)
+{
+    return volume_write(path, h_data, info, like_path);
+}
+
+ROIFT_API int roift_seeds_read(const char *path, int xsize, int ysize, int zsize, int32_t *h_seeds,
+                               int32_t *h_labels, int capacity, int *h_count, int *h_clamped)
+{
+    return seeds_read(path, xsize, ysize, zsize, h_seeds, h_labels, capacity, h_count, h_clamped);
+}
+
+ROIFT_API int roift_polarity_read(const char *path, float *h_per_class, int capacity, int *h_count)
+{
+    return polarity_read(path, h_per_class, capacity, h_count);
+}
+
+ROIFT_API int roift_volume_read_dev(roift_ctx *ctx, const char *path, void *d_dst, int dst_dtype, int flags,
+                                    int *h_shift, roift_volume_info *info)
+{
+    if (!ctx) return ROIFT_EINVAL;
+    return guarded(ctx, [&]() -> int { return volume_read_dev(*ctx, path, d_dst, dst_dtype, flags, h_shift, info); });
 }
 
 ROIFT_API int oift_fmax_gpu_run(const int *h_image, int *h_labels, int n, int xsize, int ysize, int zsize,

@@ -24,6 +24,9 @@ struct roift_ctx {
     int tune_blur_planes = 0;       // Gaussian blur: planes per workgroup (0: automatic)
     hipEvent_t ev[16] = {};
     hipGraphExec_t relax_graph = nullptr;   // batched relaxation iterations, replayed while the batch is unchanged
+    void *pinned[2] = {nullptr, nullptr};   // pinned host staging for volume reads (double-buffered)
+    size_t pinned_bytes = 0;
+    hipEvent_t io_ev[2] = {};
     std::string relax_graph_key;
     // device-event times of the last call's main kernels, ms: weight builder, OIFT
     // persistent kernel, all relaxation iterations; [3] = relaxation launches; [4] = Gaussian blur;
@@ -138,6 +141,18 @@ struct BatchVolume {
 // iteration launches as a HIP graph.
 // gft::Scene32::GaussianBlur applied `passes` times (in place when d_in == d_out).
 int run_gaussian_blur(roift_ctx &c, const Dims &d, const void *d_in, int dtype, void *d_out, int passes);
+
+// Volume, seed and polarity files (volio.hip).  Host-only entries report errors through
+// io_last_error(); the device read also sets the context's last error.
+const char *io_last_error();
+int volume_info(const char *path, roift_volume_info *info);
+int volume_read(const char *path, int32_t *out, long long n);
+int volume_write(const char *path, const int32_t *data, const roift_volume_info *info, const char *like);
+int seeds_read(const char *path, int xs, int ys, int zs, int32_t *seeds, int32_t *labels, int cap, int *count,
+               int *clamped);
+int polarity_read(const char *path, float *per_class, int cap, int *count);
+int volume_read_dev(roift_ctx &c, const char *path, void *d_dst, int dst_dtype, int flags, int *h_shift,
+                    roift_volume_info *info);
 
 // The CLI's stages around the hot path (stages.hip).
 int run_sobel(roift_ctx &c, const Dims &d, const void *d_image, int dtype, int32_t *d_grad, bool normalize);

@@ -14,6 +14,7 @@ to.  Re-run only if the inputs below change:
     python tests/golden/make_golden.py          # everything
     python tests/golden/make_golden.py blur     # only blur/
     python tests/golden/make_golden.py stages   # only stages/
+    python tests/golden/make_golden.py cli      # only cli/ (the reference CLI run on files)
 """
 import os
 import sys
@@ -201,6 +202,52 @@ def write_stages():
         print("dilation", name)
 
 
+def cli_cases():
+    """(name, volume file, volume, seed lines, polarity lines or None, argv tail, output file)."""
+    c = P.small_case((24, 28, 32), 5, per=50.0)
+    img = c.image.astype(np.int32)
+    zs, ys, xs = img.shape
+    xyz = lambda p: (int(p) % xs, (int(p) // xs) % ys, int(p) // (xs * ys))
+    base = [(*xyz(p), 0, int(lb)) for p, lb in zip(c.seeds[:27], c.seed_labels[:27])]   # object cube
+    bg = [(*xyz(p), 0, 0) for p in c.seeds[27::97][:12]]                                # a few faces
+    yield "binary_scn", "vol.scn", img, base + bg, None, ["0.5", "10", "50"], "out.scn", []
+    yield "negative_nii", "vol.nii.gz", img - 500, base + bg, None, ["-0.3", "5", "30"], "out.nii.gz", ["4"]
+    three = base[:13] + [(3, 3, 3, 0, 2), (4, 3, 3, 0, 2)] + base[13:] + bg
+    yield "perclass", "vol.nii.gz", img, three, ["3", "-0.5 0.5 0.5"], ["0.5", "5", "50"], "out.nii", ["0"]
+    odd = base[:10] + [(1, 1, 1, 0, -3), (xs + 5, 2, 2, 0, 1)] + base[10:] + bg
+    yield "clamped", "vol.scn", img, odd, None, ["0.0", "0", "80"], "out.nii.gz", ["6"]
+
+
+def write_cli():
+    import shutil
+    root = os.path.join(HERE, "cli")
+    for name, vname, vol, seeds, pol, args, oname, tail in cli_cases():
+        d = os.path.join(root, name)
+        shutil.rmtree(d, ignore_errors=True)
+        os.makedirs(d)
+        O.ref_scene_write(os.path.join(d, vname), vol)
+        with open(os.path.join(d, "seeds.txt"), "w") as f:
+            f.write(f"{len(seeds)}\n" + "".join(f"{x} {y} {z} {i} {lb}\n" for x, y, z, i, lb in seeds))
+        extra = []
+        if pol is not None:
+            with open(os.path.join(d, "pol.txt"), "w") as f:
+                f.write("\n".join(pol) + "\n")
+            extra = ["pol.txt"]
+        argv = [vname, "seeds.txt"] + args + [oname] + tail + extra
+        cwd = os.getcwd()
+        os.chdir(d)
+        try:
+            assert O.ref_cli(argv) == 0
+            labels = O.ref_scene_read(oname)
+            os.remove(oname)
+        finally:
+            os.chdir(cwd)
+        with open(os.path.join(d, "argv.txt"), "w") as f:
+            f.write(" ".join(argv) + "\n")
+        np.savez_compressed(os.path.join(d, "expected.npz"), label=labels)
+        print("cli", name, argv, np.bincount(labels.ravel()))
+
+
 def main():
     if O.ref_lib() is None:
         raise SystemExit("oracle/_ref/libroift_ref.so is not built (make -C oracle ref)")
@@ -209,6 +256,9 @@ def main():
         return
     if sys.argv[1:] == ["stages"]:
         write_stages()
+        return
+    if sys.argv[1:] == ["cli"]:
+        write_cli()
         return
     for name, c in cases():
         img = np.ascontiguousarray(c["image"], np.int32)
@@ -227,6 +277,7 @@ def main():
     write_variants()
     write_blur()
     write_stages()
+    write_cli()
 
 
 if __name__ == "__main__":
