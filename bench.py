@@ -31,8 +31,9 @@ ALG = dict(weights=2 + 3 * 2 * 2, oift=12 + 4 + 4 + 4, relax_iter=2 + 4 + 4, lab
            blur2=2 + 2,   # both Gaussian-blur passes fused: read the int16 image once, write it once
            sobel=2 + 4)   # Sobel: read the int16 image, write the int32 gradient
 # HBM bytes per voxel per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
-# workload, scaled by the same counters on a 1 GiB copy (profiles/pmc_traffic_headline256.txt).
-TRAFFIC = {"k_oift_persistent": 2704.9, "k_relax2_march": 19.74, "k_weights": 16.13}
+# workload, scaled by the same counters on a 1 GiB copy (profiles/pmc_traffic_headline256.md).
+TRAFFIC = {"k_oift_persistent": 2751.24, "k_relax2_march": 10.45, "k_weights": 16.13, "k_blur": 4.71,
+           "k_sobel": 7.34}
 
 
 def cpu_baseline(size=128, runs=3):
