@@ -298,6 +298,13 @@ ROIFT_API int roift_polarity_read(const char *path, float *h_per_class, int capa
 ROIFT_API int roift_volume_read_dev(roift_ctx *ctx, const char *path, void *d_dst, int dst_dtype, int flags,
                                     int *h_shift, roift_volume_info *info);
 
+/* The reference CLI (oiftrelax.cpp:227-432) with every stage on the GPU: argv as
+ * oiftrelax's (argv[0] = program name, then <volume> <file_seeds> <pol> <niter>
+ * <percentile> <output_file> [boundary_stride] [pol_file]); prints its progress and
+ * stage timings to stdout.  Returns 0, or 1 after printing the failing stage to stderr.
+ * roift_amd/roift_oiftrelax is this function as an executable.                        */
+ROIFT_API int roift_cli_main(int argc, char **argv);
+
 /* Host-pointer forms (copy in, run, copy out).  h_image is int32 as gft's Scene32. */
 ROIFT_API int roift_oift_fmax(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image,
                               const int32_t *h_seeds, int n_seeds, int32_t *h_label, int32_t *h_cost);

@@ -32,7 +32,7 @@ EXPORTS = (
     "roift_gaussian_blur_dev", "roift_gaussian_blur",
     "roift_sobel_dev", "roift_bfs_edt_dev", "roift_boundary_seeds_dev", "roift_dilation_conditional_dev",
     "roift_io_last_error", "roift_volume_info_read", "roift_volume_read", "roift_volume_write", "roift_seeds_read",
-    "roift_polarity_read", "roift_volume_read_dev",
+    "roift_polarity_read", "roift_volume_read_dev", "roift_cli_main",
 )
 
 
@@ -115,6 +115,7 @@ def lib():
         L.roift_seeds_read.argtypes = [C.c_char_p, ip, ip, ip, vp, vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.roift_polarity_read.argtypes = [C.c_char_p, vp, ip, C.POINTER(C.c_int)]
         L.roift_volume_read_dev.argtypes = [vp, C.c_char_p, vp, ip, ip, C.POINTER(C.c_int), vip]
+        L.roift_cli_main.argtypes = [ip, C.POINTER(C.c_char_p)]
         L.roift_sobel_dev.argtypes = [vp, pp, vp, ip, vp, ip]
         L.roift_bfs_edt_dev.argtypes = [vp, pp, vp, vp, ip, vp]
         L.roift_boundary_seeds_dev.argtypes = [vp, pp, vp, ip, ip, vp, ip, vp, ip, C.POINTER(C.c_int)]
@@ -175,6 +176,13 @@ def read_polarity(path):
     n = C.c_int(0)
     _io_check(lib().roift_polarity_read(str(path).encode(), buf.ctypes.data, buf.size, C.byref(n)))
     return buf[: n.value].copy()
+
+
+def cli_main(args):
+    """The GPU oiftrelax in this process: roift_cli_main(["oiftrelax", *args]); 0 = success."""
+    argv = ["roift_oiftrelax"] + [str(a) for a in args]
+    arr = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
+    return lib().roift_cli_main(len(argv), arr)
 
 
 def make_problem(shape, adj_r=1.0, per=0.0, per_class=None):

@@ -1,5 +1,5 @@
-// cli_main.cpp -- roift_oiftrelax: the reference CLI's argv contract (oiftrelax.cpp:227-432)
-// with every stage on the GPU through libroift.so:
+// cli_main.cpp -- roift_cli_main (in libroift.so) and roift_oiftrelax (its executable):
+// the reference CLI's argv contract (oiftrelax.cpp:227-432) with every stage on the GPU:
 //
 //   roift_oiftrelax <volume> <file_seeds> <pol> <niter> <percentile> <output_file>
 //                   [boundary_stride] [pol_file]
@@ -21,28 +21,44 @@
 
 namespace {
 
-roift_ctx *g_ctx = nullptr;
+struct CliFail {};   // a stage failed; its message is already on stderr
 
-void die(const char *what, int rc, const char *msg)
-{
-    std::fprintf(stderr, "roift_oiftrelax: %s failed (%d): %s\n", what, rc, msg);
-    std::exit(1);
-}
-
-void check(const char *what, int rc)
-{
-    if (rc != ROIFT_OK) die(what, rc, roift_last_error(g_ctx));
-}
-
-void check_io(const char *what, int rc)
-{
-    if (rc != ROIFT_OK) die(what, rc, roift_io_last_error());
-}
-
-void hip_check(const char *what, hipError_t e)
-{
-    if (e != hipSuccess) die(what, (int)e, hipGetErrorString(e));
-}
+struct Cli {
+    roift_ctx *ctx = nullptr;
+    std::vector<void *> dev;
+    int32_t *h_label = nullptr;
+    ~Cli()
+    {
+        for (void *p : dev) (void)hipFree(p);
+        if (h_label) (void)hipHostFree(h_label);
+        if (ctx) roift_destroy(ctx);
+    }
+    [[noreturn]] void die(const char *what, int rc, const char *msg)
+    {
+        std::fprintf(stderr, "roift_oiftrelax: %s failed (%d): %s\n", what, rc, msg);
+        throw CliFail{};
+    }
+    void check(const char *what, int rc)
+    {
+        if (rc != ROIFT_OK) die(what, rc, roift_last_error(ctx));
+    }
+    void check_io(const char *what, int rc)
+    {
+        if (rc != ROIFT_OK) die(what, rc, roift_io_last_error());
+    }
+    void hip_check(const char *what, hipError_t e)
+    {
+        if (e != hipSuccess) die(what, (int)e, hipGetErrorString(e));
+    }
+    template <typename T>
+    T *dmalloc(size_t bytes)
+    {
+        void *p = nullptr;
+        hip_check("hipMalloc", hipMalloc(&p, bytes));
+        dev.push_back(p);
+        return (T *)p;
+    }
+};
 
 struct Timer {
     std::vector<std::pair<std::string, double>> ev;
@@ -68,23 +84,20 @@ void usage()
     std::printf("\t pol_file........ Per-class polarity file (optional): <n_labels> then pol_0 ... pol_{n-1}\n");
 }
 
-}  // namespace
-
-int main(int argc, char **argv)
+int run(Cli &cl, int argc, char **argv)
 {
-    if (argc < 7) {
-        usage();
-        return 0;
-    }
+    auto check = [&](const char *w, int rc) { cl.check(w, rc); };
+    auto check_io = [&](const char *w, int rc) { cl.check_io(w, rc); };
+    auto hip_check = [&](const char *w, hipError_t e) { cl.hip_check(w, e); };
     Timer tm;
-    check("roift_create", roift_create(&g_ctx, 0));
+    check("roift_create", roift_create(&cl.ctx, 0));
+    roift_ctx *g_ctx = cl.ctx;
 
     // volume (the CLI shifts a negative minimum to 0: oiftrelax.cpp:263-268)
     roift_volume_info info{};
     check_io("reading the volume header", roift_volume_info_read(argv[1], &info));
     const long long n = (long long)info.xsize * info.ysize * info.zsize;
-    void *d_img = nullptr;
-    hip_check("hipMalloc", hipMalloc(&d_img, (size_t)n * 4));
+    void *d_img = cl.dmalloc<void>((size_t)n * 4);
     int dtype = ROIFT_I16, shift = 0;
     if (roift_volume_read_dev(g_ctx, argv[1], d_img, ROIFT_I16, ROIFT_READ_SHIFT_MIN, &shift, &info) != ROIFT_OK) {
         dtype = ROIFT_I32;   // values beyond int16
@@ -118,14 +131,13 @@ int main(int argc, char **argv)
     check_io("reading the seeds", roift_seeds_read(argv[2], info.xsize, info.ysize, info.zsize, seeds.data(),
                                                    slab.data(), count, &count, &clamped));
     if (clamped) std::printf("Warning: negative seed labels were clamped to 0 (background).\n");
-    int32_t *h_label = nullptr;
-    hip_check("hipHostMalloc", hipHostMalloc((void **)&h_label, (size_t)n * 4, hipHostMallocDefault));
+    hip_check("hipHostMalloc", hipHostMalloc((void **)&cl.h_label, (size_t)n * 4, hipHostMallocDefault));
+    int32_t *h_label = cl.h_label;
     std::fill(h_label, h_label + n, -1);
     for (int i = 0; i < count; i++) h_label[seeds[i]] = slab[i];
-    int32_t *d_label = nullptr, *d_seeds = nullptr;
     const int face_cap = 2 * (info.xsize * info.ysize + info.ysize * info.zsize + info.xsize * info.zsize) + 8;
-    hip_check("hipMalloc", hipMalloc((void **)&d_label, (size_t)n * 4));
-    hip_check("hipMalloc", hipMalloc((void **)&d_seeds, (size_t)(count + face_cap) * 4));
+    int32_t *d_label = cl.dmalloc<int32_t>((size_t)n * 4);
+    int32_t *d_seeds = cl.dmalloc<int32_t>((size_t)(count + face_cap) * 4);
     hip_check("copy labels", hipMemcpy(d_label, h_label, (size_t)n * 4, hipMemcpyHostToDevice));
     if (count > 0) hip_check("copy seeds", hipMemcpy(d_seeds, seeds.data(), (size_t)count * 4, hipMemcpyHostToDevice));
     roift_problem pb{info.xsize, info.ysize, info.zsize, 1.0f, 0, 0.0f, nullptr, -1};
@@ -175,10 +187,23 @@ int main(int argc, char **argv)
 
     std::printf("Stage timings (ms):\n");
     for (auto &e : tm.ev) std::printf("  %-28s %10.3f\n", e.first.c_str(), e.second);
-    (void)hipHostFree(h_label);
-    (void)hipFree(d_img);
-    (void)hipFree(d_label);
-    (void)hipFree(d_seeds);
-    roift_destroy(g_ctx);
+    std::fflush(stdout);
     return 0;
+}
+
+}  // namespace
+
+// 0 on success (and for the usage text), 1 on any failure (the message goes to stderr).
+extern "C" __attribute__((visibility("default"))) int roift_cli_main(int argc, char **argv)
+{
+    if (argc < 7) {
+        usage();
+        return 0;
+    }
+    Cli cl;
+    try {
+        return run(cl, argc, argv);
+    } catch (const CliFail &) {
+        return 1;
+    }
 }

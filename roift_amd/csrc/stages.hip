@@ -244,6 +244,46 @@ __global__ __launch_bounds__(256) void k_edt_pass(int32_t *__restrict__ dist, Di
     }
 }
 
+// The x pass: one wavefront per row, as min-plus scans over 64-voxel chunks.  Forward,
+// F(x) = x + min_{x' <= x} (d(x') - x'); backward on F, D(x) = -x + min_{x' >= x} (F(x') + x'),
+// which is min_{x'} d(x') + |x - x'| -- the same values as the sequential sweeps, clamped
+// at EDT_INF the same way.
+__global__ __launch_bounds__(256) void k_edt_rows(int32_t *__restrict__ dist, Dims d)
+{
+    const long long rows = (long long)d.ys * d.zs;
+    const int lane = __lane_id();
+    constexpr int BIG = 0x7FFFFFFF - (1 << 16);   // above every d - x and F + x
+    for (long long r = gtid() >> 6; r < rows; r += gstride() >> 6) {
+        int32_t *row = dist + r * d.xs;
+        int carry = BIG;
+        for (int x0 = 0; x0 < d.xs; x0 += 64) {
+            const int x = x0 + lane;
+            int v = x < d.xs ? row[x] - x : BIG;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(v, o);
+                if (lane >= o) v = min(v, t);
+            }
+            v = min(v, carry);
+            carry = __shfl(v, 63);
+            if (x < d.xs) row[x] = min(x + v, EDT_INF);
+        }
+        carry = BIG;
+        for (int x0 = ((d.xs - 1) / 64) * 64; x0 >= 0; x0 -= 64) {
+            const int x = x0 + lane;
+            int v = x < d.xs ? row[x] + x : BIG;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_down(v, o);
+                if (lane + o < 64) v = min(v, t);
+            }
+            v = min(v, carry);
+            carry = __shfl(v, 0);
+            if (x < d.xs) row[x] = min(v - x, EDT_INF);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // Boundary seeds
 // ---------------------------------------------------------------------------------
@@ -333,6 +373,32 @@ __global__ __launch_bounds__(256) void k_dil_border(const int32_t *__restrict__ 
             }
         }
         border[p] = b;
+    }
+}
+
+// Spheric(1.0) border, z-marching: a workgroup owns a 64 x 4 column tile and a chunk of
+// planes, each thread keeps its column's labels at z-1, z, z+1 in registers and reads the
+// four in-plane neighbours (rows the workgroup shares in cache).
+constexpr int DB_X = 64, DB_Y = 4, DB_Z = 32;
+__global__ __launch_bounds__(DB_X *DB_Y) void k_dil_border6(const int32_t *__restrict__ label, Dims d,
+                                                          uint8_t *__restrict__ border)
+{
+    const int x = blockIdx.x * DB_X + threadIdx.x, y = blockIdx.y * DB_Y + threadIdx.y;
+    const int z0 = blockIdx.z * DB_Z, z1 = min(d.zs, z0 + DB_Z);
+    if (x >= d.xs || y >= d.ys) return;   // no workgroup barrier below
+    const size_t xy = (size_t)d.xs * d.ys, col = (size_t)y * d.xs + x;
+    int lm = z0 > 0 ? label[(size_t)(z0 - 1) * xy + col] : -1;
+    int lc = label[(size_t)z0 * xy + col];
+    for (int z = z0; z < z1; z++) {
+        const size_t p = (size_t)z * xy + col;
+        const int lp = z + 1 < d.zs ? label[p + xy] : -1;
+        bool b = false;
+        if (lc == 0)
+            b = lm == 1 || lp == 1 || (x > 0 && label[p - 1] == 1) || (x + 1 < d.xs && label[p + 1] == 1) ||
+                (y > 0 && label[p - d.xs] == 1) || (y + 1 < d.ys && label[p + d.xs] == 1);
+        border[p] = b;
+        lm = lc;
+        lc = lp;
     }
 }
 
@@ -444,7 +510,7 @@ int run_bfs_edt(roift_ctx &c, const Dims &d, const int32_t *d_label, const int32
                            d.n, d_dist);
         ROIFT_HIP_CHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_edt_pass<0>, dim3(grid_for((long long)d.ys * d.zs)), dim3(256), 0, c.stream, d_dist, d, false);
+    hipLaunchKernelGGL(k_edt_rows, dim3(grid_for((long long)d.ys * d.zs * 64)), dim3(256), 0, c.stream, d_dist, d);
     hipLaunchKernelGGL(k_edt_pass<1>, dim3(grid_for((long long)d.xs * d.zs)), dim3(256), 0, c.stream, d_dist, d, false);
     hipLaunchKernelGGL(k_edt_pass<2>, dim3(grid_for((long long)d.xs * d.ys)), dim3(256), 0, c.stream, d_dist, d, true);
     ROIFT_HIP_CHECK(hipGetLastError());
@@ -502,8 +568,12 @@ int run_dilation_conditional(roift_ctx &c, const Dims &d, float radius, const vo
     int32_t *pos = buf<int32_t>(c, "dil.pos", d.n);
     uint32_t *key = buf<uint32_t>(c, "dil.key", d.n);
     ev_record(c, 14);
-    hipLaunchKernelGGL(k_dil_border, dim3(grid_for(d.n, 256, 8192)), dim3(256), 0, c.stream, (const int32_t *)d_label,
-                       d, A, border);
+    if (A.n == 7)
+        hipLaunchKernelGGL(k_dil_border6, dim3((d.xs + DB_X - 1) / DB_X, (d.ys + DB_Y - 1) / DB_Y, (d.zs + DB_Z - 1) / DB_Z),
+                           dim3(DB_X, DB_Y), 0, c.stream, (const int32_t *)d_label, d, border);
+    else
+        hipLaunchKernelGGL(k_dil_border, dim3(grid_for(d.n, 256, 8192)), dim3(256), 0, c.stream,
+                           (const int32_t *)d_label, d, A, border);
     ROIFT_HIP_CHECK(hipGetLastError());
     const uint32_t nb = dtype == ROIFT_I16 ? border_list<int16_t>(c, d, d_image, border, pos, key)
                                            : border_list<int32_t>(c, d, d_image, border, pos, key);

@@ -21,20 +21,33 @@ CASES = sorted(os.path.dirname(p) for p in glob.glob(os.path.join(CLI, "*", "arg
 
 @pytest.mark.parametrize("case", CASES, ids=os.path.basename)
 def test_cli_matches_reference_cli(case, tmp_path):
-    assert os.path.exists(EXE), "roift_oiftrelax is not built (make -C roift_amd/csrc)"
+    """roift_cli_main, the body of roift_oiftrelax, run in this process (one process keeps
+    the GPU's cooperative launches; the executable is the same function)."""
     for f in os.listdir(case):
         shutil.copy(os.path.join(case, f), tmp_path)
     argv = open(os.path.join(case, "argv.txt")).read().split()
     print("case", os.path.basename(case), argv, flush=True)
-    r = subprocess.run([EXE] + argv, cwd=tmp_path, capture_output=True, text=True, timeout=120)
-    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
-    assert r.returncode == 0
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        rc = R.cli_main(argv)
+    finally:
+        os.chdir(cwd)
+    assert rc == 0
     got, _ = R.read_volume(tmp_path / argv[5])
     want = np.load(os.path.join(case, "expected.npz"))["label"]
     assert np.array_equal(got, want)
 
 
-def test_cli_usage_without_arguments():
+def test_cli_errors_return_one(tmp_path):
+    shutil.copy(os.path.join(CASES[0], "vol.scn"), tmp_path)
+    assert R.cli_main([str(tmp_path / "vol.scn"), str(tmp_path / "missing.txt"), 0.5, 1, 50,
+                       str(tmp_path / "out.scn")]) == 1
+    assert R.cli_main([str(tmp_path / "missing.scn"), "s.txt", 0.5, 1, 50, str(tmp_path / "out.scn")]) == 1
+
+
+def test_cli_executable_usage():
+    assert os.path.exists(EXE), "roift_oiftrelax is not built (make -C roift_amd/csrc)"
     r = subprocess.run([EXE], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "usage" in r.stdout
 
