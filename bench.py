@@ -128,10 +128,11 @@ def adjoint_bench(ctx, count=16, size=128, niter=50, ckpt=10):
         sigma = float(np.float32(np.sqrt(c.image.astype(np.float64).var())))
         vols.append((img, seeds, lab, scale, sigma, target))
 
-    def run():
+    def run(fp64=False, nv=count):
         out = []
-        for img, seeds, lab, scale, sigma, target in vols:
-            out.append(ctx.relax_adjoint(img, seeds, lab, 1, niter, scale, sigma, 0.5, target, ckpt_every=ckpt)[0])
+        for img, seeds, lab, scale, sigma, target in vols[:nv]:
+            out.append(ctx.relax_adjoint(img, seeds, lab, 1, niter, scale, sigma, 0.5, target, ckpt_every=ckpt,
+                                         fp64=fp64)[0])
         return out
 
     run()
@@ -141,10 +142,19 @@ def adjoint_bench(ctx, count=16, size=128, niter=50, ckpt=10):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     nvox = count * size ** 3
+    # the FP64 form (the one checked against FP64 finite differences to 1e-4) on 4 of them
+    n64 = min(4, count)
+    run(True, n64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(True, n64)
+    torch.cuda.synchronize()
+    dt64 = time.perf_counter() - t0
     return {"workload": f"{count} x {size}^3, differentiable relax {niter} iterations fwd + adjoint, "
                         f"checkpoint every {ckpt}, BCE vs blurred truth",
             "voxels": nvox, "fwd_bwd_mvox_s": round(nvox / dt / 1e6, 2), "ms": round(dt * 1e3, 1),
-            "mean_loss": round(float(np.mean(losses)), 6)}
+            "mean_loss": round(float(np.mean(losses)), 6),
+            "fp64_fwd_bwd_mvox_s": round(n64 * size ** 3 / dt64 / 1e6, 2)}
 
 
 def main():

@@ -211,6 +211,15 @@ ROIFT_API int roift_relax_adjoint_dev(roift_ctx *ctx, const roift_problem *pb, c
                                       int niter, int ckpt_every, const float *d_scale, float sigma, float alpha,
                                       const float *d_target, float eps, double *h_loss, float *d_grad_scale,
                                       double *h_grad_sigma, double *h_grad_alpha, float *d_memb);
+/* The same in FP64 (sigma, alpha, eps, states, arithmetic, d_grad_scale and d_memb are
+ * double; the scale map and the target stay FP32 inputs): the gradient a caller checks
+ * against FP64 finite differences (1e-4 relative).                                     */
+ROIFT_API int roift_relax_adjoint64_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image,
+                                        int image_dtype, const int32_t *d_seeds, int n_seeds, const int32_t *d_label,
+                                        int lab_obj, int niter, int ckpt_every, const float *d_scale, double sigma,
+                                        double alpha, const float *d_target, double eps, double *h_loss,
+                                        double *d_grad_scale, double *h_grad_sigma, double *h_grad_alpha,
+                                        double *d_memb);
 
 /* ---- preprocessing ------------------------------------------------------------ */
 /* gft::Scene32::GaussianBlur (gft_filtering3.cpp:385-397) applied `passes` times

@@ -32,7 +32,7 @@ EXPORTS = (
     "roift_gaussian_blur_dev", "roift_gaussian_blur",
     "roift_sobel_dev", "roift_bfs_edt_dev", "roift_boundary_seeds_dev", "roift_dilation_conditional_dev",
     "roift_io_last_error", "roift_volume_info_read", "roift_volume_read", "roift_volume_write", "roift_seeds_read",
-    "roift_polarity_read", "roift_volume_read_dev", "roift_cli_main",
+    "roift_polarity_read", "roift_volume_read_dev", "roift_cli_main", "roift_relax_adjoint64_dev",
 )
 
 
@@ -104,6 +104,9 @@ def lib():
         L.roift_batch_pipeline_dev.argtypes = [vp, ip, i32p, fl, fl, vp, ip, vp, i32p, vp, ip, vp, ip]
         L.roift_relax_adjoint_dev.argtypes = [vp, pp, vp, ip, vp, ip, vp, ip, ip, ip, vp, fl, fl, vp, fl, dp, vp, dp,
                                               dp, vp]
+        dbl = C.c_double
+        L.roift_relax_adjoint64_dev.argtypes = [vp, pp, vp, ip, vp, ip, vp, ip, ip, ip, vp, dbl, dbl, vp, dbl, dp, vp,
+                                                dp, dp, vp]
         L.roift_gaussian_blur_dev.argtypes = [vp, pp, vp, ip, vp, ip]
         L.roift_gaussian_blur.argtypes = [vp, pp, vp, vp, ip]
         L.roift_io_last_error.argtypes = []
@@ -491,17 +494,20 @@ class Context:
         return out
 
     def relax_adjoint(self, image, seeds, label, lab_obj, niter, scale, sigma, alpha, target, ckpt_every=10,
-                      adj_r=1.0, eps=1e-6, want_memb=False):
-        """Differentiable relaxation + adjoint: returns (loss, grad_scale, grad_sigma, grad_alpha, f or None)."""
+                      adj_r=1.0, eps=1e-6, want_memb=False, fp64=False):
+        """Differentiable relaxation + adjoint: returns (loss, grad_scale, grad_sigma, grad_alpha, f or None);
+        fp64 runs states and gradients in double (grad_scale and f are float64 then)."""
         import torch
         pb, _ = make_problem(image.shape, adj_r, 0.0)
         seeds = seeds.to(device=label.device, dtype=torch.int32).contiguous()
         sc = scale.to(torch.float32).contiguous()
         tg = target.to(torch.float32).contiguous()
-        gsc = torch.empty_like(sc)
-        memb = torch.empty_like(sc) if want_memb else None
+        rt = torch.float64 if fp64 else torch.float32
+        gsc = torch.empty(tuple(sc.shape), dtype=rt, device=sc.device)
+        memb = torch.empty(tuple(sc.shape), dtype=rt, device=sc.device) if want_memb else None
         loss, gs, ga = C.c_double(), C.c_double(), C.c_double()
-        self._check(self._L.roift_relax_adjoint_dev(
+        fn = self._L.roift_relax_adjoint64_dev if fp64 else self._L.roift_relax_adjoint_dev
+        self._check(fn(
             self._h, C.byref(pb), image.data_ptr(), _dtype_code(image), seeds.data_ptr(), int(seeds.numel()),
             label.data_ptr(), int(lab_obj), int(niter), int(ckpt_every), sc.data_ptr(), float(sigma), float(alpha),
             tg.data_ptr(), float(eps), C.byref(loss), gsc.data_ptr(), C.byref(gs), C.byref(ga),

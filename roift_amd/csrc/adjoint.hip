@@ -12,36 +12,49 @@
 //   pass 1 (per p):  B_p = sum_q w_pq,  c_p = lambda'_p / B_p  (the self term when B_p = 0)
 //   pass 2 (per q):  lambda_q = sum_{p ~ q} c_p (w_pq + (f_q - f'_p) dw_pq/df_q)
 //                    and the parameter gradients, gathered at q.
-// States are FP32; the scalar gradients and the loss accumulate in FP64.
+// States and arithmetic are FP32 (roift_relax_adjoint_dev) or FP64 (roift_relax_adjoint64_dev,
+// the same kernels instantiated for double); the scalar gradients and the loss always
+// accumulate in FP64.  The learnable scale map and the target are FP32 inputs in both.
+#include <type_traits>
+
 #include "ctx.hpp"
 
 namespace roift {
 namespace {
 
+template <typename R>
 struct DiffArgs {
     Dims d;
-    float inv_s2;      // 1 / sigma^2
-    float sigma;
-    float alpha;
-    float Dpq[27];
+    R inv_s2;      // 1 / sigma^2
+    R sigma;
+    R alpha;
+    R Dpq[27];
 };
 
-template <int NADJ, typename IT>
-__device__ inline float dweight(const IT *img, const float *scale, const DiffArgs &a, int k, long long p,
-                                long long q, float fq, float &g, float &sg, float &pi)
+template <typename R>
+__device__ inline R rexp(R x)
 {
-    const float d = (float)((int)img[p] - (int)img[q]);
-    g = expf(-(d * d) * a.inv_s2);
-    sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
-    pi = 1.0f + a.alpha * sg * (2.0f * fq - 1.0f);
-    return scale[q] * g * pi / a.Dpq[k];
+    if constexpr (std::is_same<R, float>::value) return expf(x);
+    else return exp(x);
 }
 
-__global__ void k_diff_init(const int32_t *__restrict__ label, long long n, int lab_obj, float *__restrict__ f,
+template <int NADJ, typename IT, typename R>
+__device__ inline R dweight(const IT *img, const float *scale, const DiffArgs<R> &a, int k, long long p, long long q,
+                            R fq, R &g, R &sg, R &pi)
+{
+    const R d = (R)((int)img[p] - (int)img[q]);
+    g = rexp<R>(-(d * d) * a.inv_s2);
+    sg = d > (R)0 ? (R)1 : (d < (R)0 ? (R)-1 : (R)0);
+    pi = (R)1 + a.alpha * sg * ((R)2 * fq - (R)1);
+    return (R)scale[q] * g * pi / a.Dpq[k];
+}
+
+template <typename R>
+__global__ void k_diff_init(const int32_t *__restrict__ label, long long n, int lab_obj, R *__restrict__ f,
                             uint8_t *__restrict__ seed)
 {
     for (long long p = gtid(); p < n; p += gstride()) {
-        f[p] = (label[p] == lab_obj) ? 0.0f : 1.0f;
+        f[p] = (label[p] == lab_obj) ? (R)0 : (R)1;
         seed[p] = 0;
     }
 }
@@ -56,79 +69,79 @@ __global__ void k_diff_seeds(const int32_t *__restrict__ S, int ns, long long n,
     }
 }
 
-template <int NADJ, typename IT>
+template <int NADJ, typename IT, typename R>
 __global__ __launch_bounds__(256) void k_diff_fwd(const IT *__restrict__ img, const float *__restrict__ scale,
-                                                   DiffArgs a, const uint8_t *__restrict__ seed,
-                                                   const float *__restrict__ fin, float *__restrict__ fout)
+                                                   DiffArgs<R> a, const uint8_t *__restrict__ seed,
+                                                   const R *__restrict__ fin, R *__restrict__ fout)
 {
     using A = Adjacency<NADJ>;
     const Dims d = a.d;
     const long long xy = (long long)d.xs * d.ys;
     for (long long p = gtid(); p < d.n; p += gstride()) {
-        float v = fin[p];
+        R v = fin[p];
         if (!seed[p]) {
             const int x = (int)(p % d.xs), y = (int)((p / d.xs) % d.ys), z = (int)(p / xy);
-            float num = 0.0f, den = 0.0f;
+            R num = 0, den = 0;
 #pragma unroll
             for (int k = 1; k < NADJ; k++) {
                 const int qx = x + A::dx(k), qy = y + A::dy(k), qz = z + A::dz(k);
                 if (!in_volume(d, qx, qy, qz)) continue;
                 const long long q = qx + (long long)qy * d.xs + (long long)qz * xy;
-                const float fq = fin[q];
-                float g, sg, pi;
-                const float w = dweight<NADJ, IT>(img, scale, a, k, p, q, fq, g, sg, pi);
+                const R fq = fin[q];
+                R g, sg, pi;
+                const R w = dweight<NADJ, IT, R>(img, scale, a, k, p, q, fq, g, sg, pi);
                 num += w * fq;
                 den += w;
             }
-            if (den != 0.0f) v = num / den;
+            if (den != (R)0) v = num / den;
         }
         fout[p] = v;
     }
 }
 
 // Loss and its gradient with respect to the final f.
-__global__ __launch_bounds__(256) void k_diff_loss(const float *__restrict__ f, const float *__restrict__ target,
-                                                    long long n, float eps, float *__restrict__ lam,
-                                                    double *__restrict__ loss)
+template <typename R>
+__global__ __launch_bounds__(256) void k_diff_loss(const R *__restrict__ f, const float *__restrict__ target,
+                                                    long long n, R eps, R *__restrict__ lam, double *__restrict__ loss)
 {
     double acc = 0.0;
     for (long long p = gtid(); p < n; p += gstride()) {
-        float o = 1.0f - f[p];
+        R o = (R)1 - f[p];
         bool clamped = false;
         if (o < eps) { o = eps; clamped = true; }
-        if (o > 1.0f - eps) { o = 1.0f - eps; clamped = true; }
+        if (o > (R)1 - eps) { o = (R)1 - eps; clamped = true; }
         const float t = target[p];
         acc -= (double)t * log((double)o) + (1.0 - (double)t) * log(1.0 - (double)o);
-        lam[p] = clamped ? 0.0f : (float)(((double)t / o - (1.0 - (double)t) / (1.0 - (double)o)) / (double)n);
+        lam[p] = clamped ? (R)0 : (R)(((double)t / o - (1.0 - (double)t) / (1.0 - (double)o)) / (double)n);
     }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if (__lane_id() == 0) atomicAdd(loss, acc / (double)n);
+    acc = block_reduce(acc, OpAdd());
+    if (threadIdx.x == 0) atomicAdd(loss, acc / (double)n);
 }
 
-template <int NADJ, typename IT>
+template <int NADJ, typename IT, typename R>
 __global__ __launch_bounds__(256) void k_diff_bwd1(const IT *__restrict__ img, const float *__restrict__ scale,
-                                                    DiffArgs a, const uint8_t *__restrict__ seed,
-                                                    const float *__restrict__ fin, const float *__restrict__ lam,
-                                                    float *__restrict__ c, float *__restrict__ lam_out)
+                                                    DiffArgs<R> a, const uint8_t *__restrict__ seed,
+                                                    const R *__restrict__ fin, const R *__restrict__ lam,
+                                                    R *__restrict__ c, R *__restrict__ lam_out)
 {
     using A = Adjacency<NADJ>;
     const Dims d = a.d;
     const long long xy = (long long)d.xs * d.ys;
     for (long long p = gtid(); p < d.n; p += gstride()) {
-        float cp = 0.0f, self = 0.0f;
-        const float l = lam[p];
-        if (!seed[p] && l != 0.0f) {
+        R cp = 0, self = 0;
+        const R l = lam[p];
+        if (!seed[p] && l != (R)0) {
             const int x = (int)(p % d.xs), y = (int)((p / d.xs) % d.ys), z = (int)(p / xy);
-            float den = 0.0f;
+            R den = 0;
 #pragma unroll
             for (int k = 1; k < NADJ; k++) {
                 const int qx = x + A::dx(k), qy = y + A::dy(k), qz = z + A::dz(k);
                 if (!in_volume(d, qx, qy, qz)) continue;
                 const long long q = qx + (long long)qy * d.xs + (long long)qz * xy;
-                float g, sg, pi;
-                den += dweight<NADJ, IT>(img, scale, a, k, p, q, fin[q], g, sg, pi);
+                R g, sg, pi;
+                den += dweight<NADJ, IT, R>(img, scale, a, k, p, q, fin[q], g, sg, pi);
             }
-            if (den != 0.0f) cp = l / den;
+            if (den != (R)0) cp = l / den;
             else self = l;
         }
         c[p] = cp;
@@ -136,100 +149,99 @@ __global__ __launch_bounds__(256) void k_diff_bwd1(const IT *__restrict__ img, c
     }
 }
 
-template <int NADJ, typename IT>
+template <int NADJ, typename IT, typename R>
 __global__ __launch_bounds__(256) void k_diff_bwd2(const IT *__restrict__ img, const float *__restrict__ scale,
-                                                    DiffArgs a, const float *__restrict__ fin,
-                                                    const float *__restrict__ fout, const float *__restrict__ c,
-                                                    float *__restrict__ lam_out, float *__restrict__ gscale,
+                                                    DiffArgs<R> a, const R *__restrict__ fin,
+                                                    const R *__restrict__ fout, const R *__restrict__ c,
+                                                    R *__restrict__ lam_out, R *__restrict__ gscale,
                                                     double *__restrict__ gpar)
 {
     using A = Adjacency<NADJ>;
     const Dims d = a.d;
     const long long xy = (long long)d.xs * d.ys;
     double gsig = 0.0, galp = 0.0;
-    const float s3 = a.sigma * a.sigma * a.sigma;
+    const R s3 = a.sigma * a.sigma * a.sigma;
     for (long long q = gtid(); q < d.n; q += gstride()) {
         const int x = (int)(q % d.xs), y = (int)((q / d.xs) % d.ys), z = (int)(q / xy);
-        const float fq = fin[q];
-        float lq = lam_out[q], gs = 0.0f;
+        const R fq = fin[q];
+        R lq = lam_out[q], gs = 0;
 #pragma unroll
         for (int k = 1; k < NADJ; k++) {
             const int px = x + A::dx(k), py = y + A::dy(k), pz = z + A::dz(k);
             if (!in_volume(d, px, py, pz)) continue;
             const long long p = px + (long long)py * d.xs + (long long)pz * xy;
-            const float cp = c[p];
-            if (cp == 0.0f) continue;
+            const R cp = c[p];
+            if (cp == (R)0) continue;
             // the arc p -> q is p's neighbour in the opposite direction; Dpq is symmetric
-            float g, sg, pi;
-            const float w = dweight<NADJ, IT>(img, scale, a, k, p, q, fq, g, sg, pi);
-            const float common = fq - fout[p];
-            const float dwdf = scale[q] * g * a.alpha * sg * 2.0f / a.Dpq[k];
+            R g, sg, pi;
+            const R w = dweight<NADJ, IT, R>(img, scale, a, k, p, q, fq, g, sg, pi);
+            const R common = fq - fout[p];
+            const R dwdf = (R)scale[q] * g * a.alpha * sg * (R)2 / a.Dpq[k];
             lq += cp * (w + common * dwdf);
             gs += cp * common * g * pi / a.Dpq[k];
-            const float dd = (float)((int)img[p] - (int)img[q]);
+            const R dd = (R)((int)img[p] - (int)img[q]);
             gsig += (double)(cp * common) * (double)w * 2.0 * (double)dd * (double)dd / (double)s3;
-            galp += (double)(cp * common) * (double)(scale[q] * g * sg * (2.0f * fq - 1.0f) / a.Dpq[k]);
+            galp += (double)(cp * common) * (double)((R)scale[q] * g * sg * ((R)2 * fq - (R)1) / a.Dpq[k]);
         }
         lam_out[q] = lq;
         gscale[q] += gs;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        gsig += __shfl_xor(gsig, o);
-        galp += __shfl_xor(galp, o);
-    }
-    if (__lane_id() == 0) {
+    gsig = block_reduce(gsig, OpAdd());
+    galp = block_reduce(galp, OpAdd());
+    if (threadIdx.x == 0) {
         atomicAdd(&gpar[0], gsig);
         atomicAdd(&gpar[1], galp);
     }
 }
 
-template <int NADJ, typename IT>
-void diff_run(roift_ctx &c, const DiffArgs &a, const void *d_image, const int32_t *d_seeds, int n_seeds,
+template <int NADJ, typename IT, typename R>
+void diff_run(roift_ctx &c, const DiffArgs<R> &a, const void *d_image, const int32_t *d_seeds, int n_seeds,
               const int32_t *d_label, int lab_obj, int niter, int ckpt, const float *d_scale, const float *d_target,
-              float eps, double *h_loss, float *d_gscale, double *h_gsig, double *h_galp, float *d_memb)
+              R eps, double *h_loss, R *d_gscale, double *h_gsig, double *h_galp, R *d_memb)
 {
     hipStream_t st = c.stream;
     const long long n = a.d.n;
+    const size_t rb = sizeof(R) * (size_t)n;
     const IT *img = (const IT *)d_image;
     const unsigned G = grid_for(n, 256, 8192);
     const int nseg = niter > 0 ? (niter + ckpt - 1) / ckpt : 0;
     uint8_t *seed = buf<uint8_t>(c, "diff.seed", n);
-    float *ck = buf<float>(c, "diff.ckpt", (size_t)std::max(nseg, 1) * n);
-    float *seg = buf<float>(c, "diff.seg", (size_t)(ckpt + 1) * n);
-    float *lam = buf<float>(c, "diff.lam", n), *lam2 = buf<float>(c, "diff.lam2", n);
-    float *cc = buf<float>(c, "diff.c", n);
+    R *ck = buf<R>(c, "diff.ckpt", (size_t)std::max(nseg, 1) * n);
+    R *seg = buf<R>(c, "diff.seg", (size_t)(ckpt + 1) * n);
+    R *lam = buf<R>(c, "diff.lam", n), *lam2 = buf<R>(c, "diff.lam2", n);
+    R *cc = buf<R>(c, "diff.c", n);
     double *dacc = buf<double>(c, "diff.acc", 4);
     int *bad = reinterpret_cast<int *>(dacc + 3);
     const double zero[4] = {0, 0, 0, 0};
     ROIFT_HIP_CHECK(hipMemcpyAsync(dacc, zero, sizeof(zero), hipMemcpyHostToDevice, st));
     // forward, keeping the state at the start of every segment
-    hipLaunchKernelGGL(k_diff_init, dim3(G), dim3(256), 0, st, d_label, n, lab_obj, seg, seed);
+    hipLaunchKernelGGL(k_diff_init<R>, dim3(G), dim3(256), 0, st, d_label, n, lab_obj, seg, seed);
     if (n_seeds > 0)
         hipLaunchKernelGGL(k_diff_seeds, dim3(grid_for(n_seeds)), dim3(256), 0, st, d_seeds, n_seeds, n, seed, bad);
-    float *cur = seg, *nxt = seg + n;
+    R *cur = seg, *nxt = seg + n;
     for (int it = 0; it < niter; it++) {
         if (it % ckpt == 0)
-            ROIFT_HIP_CHECK(hipMemcpyAsync(ck + (size_t)(it / ckpt) * n, cur, 4 * n, hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL((k_diff_fwd<NADJ, IT>), dim3(G), dim3(256), 0, st, img, d_scale, a, seed, cur, nxt);
+            ROIFT_HIP_CHECK(hipMemcpyAsync(ck + (size_t)(it / ckpt) * n, cur, rb, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL((k_diff_fwd<NADJ, IT, R>), dim3(G), dim3(256), 0, st, img, d_scale, a, seed, cur, nxt);
         std::swap(cur, nxt);
     }
-    if (d_memb) ROIFT_HIP_CHECK(hipMemcpyAsync(d_memb, cur, 4 * n, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_diff_loss, dim3(G), dim3(256), 0, st, cur, d_target, n, eps, lam, dacc);
-    fill<float>(c, d_gscale, n, 0.0f);
+    if (d_memb) ROIFT_HIP_CHECK(hipMemcpyAsync(d_memb, cur, rb, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_diff_loss<R>, dim3(G), dim3(256), 0, st, cur, d_target, n, eps, lam, dacc);
+    fill<R>(c, d_gscale, n, (R)0);
     ROIFT_HIP_CHECK(hipGetLastError());
     // backward, segment by segment
     for (int s = nseg - 1; s >= 0; s--) {
         const int k0 = s * ckpt, k1 = std::min(niter, k0 + ckpt), m = k1 - k0;
-        ROIFT_HIP_CHECK(hipMemcpyAsync(seg, ck + (size_t)s * n, 4 * n, hipMemcpyDeviceToDevice, st));
+        ROIFT_HIP_CHECK(hipMemcpyAsync(seg, ck + (size_t)s * n, rb, hipMemcpyDeviceToDevice, st));
         for (int j = 0; j < m; j++)
-            hipLaunchKernelGGL((k_diff_fwd<NADJ, IT>), dim3(G), dim3(256), 0, st, img, d_scale, a, seed,
+            hipLaunchKernelGGL((k_diff_fwd<NADJ, IT, R>), dim3(G), dim3(256), 0, st, img, d_scale, a, seed,
                                seg + (size_t)j * n, seg + (size_t)(j + 1) * n);
         for (int j = m - 1; j >= 0; j--) {
-            const float *fin = seg + (size_t)j * n, *fout = seg + (size_t)(j + 1) * n;
-            hipLaunchKernelGGL((k_diff_bwd1<NADJ, IT>), dim3(G), dim3(256), 0, st, img, d_scale, a, seed, fin, lam,
-                               cc, lam2);
-            hipLaunchKernelGGL((k_diff_bwd2<NADJ, IT>), dim3(G), dim3(256), 0, st, img, d_scale, a, fin, fout, cc,
-                               lam2, d_gscale, dacc + 1);
+            const R *fin = seg + (size_t)j * n, *fout = seg + (size_t)(j + 1) * n;
+            hipLaunchKernelGGL((k_diff_bwd1<NADJ, IT, R>), dim3(G), dim3(256), 0, st, img, d_scale, a, seed, fin,
+                               lam, cc, lam2);
+            hipLaunchKernelGGL((k_diff_bwd2<NADJ, IT, R>), dim3(G), dim3(256), 0, st, img, d_scale, a, fin, fout,
+                               cc, lam2, d_gscale, dacc + 1);
             std::swap(lam, lam2);
         }
     }
@@ -243,41 +255,58 @@ void diff_run(roift_ctx &c, const DiffArgs &a, const void *d_image, const int32_
     if (h_galp) *h_galp = h[2];
 }
 
+template <typename R>
+void diff_dispatch(roift_ctx &c, int nadj, R sigma, R alpha, const Dims &d, const void *d_image,
+                   int image_dtype, const int32_t *d_seeds, int n_seeds, const int32_t *d_label, int lab_obj,
+                   int niter, int ckpt, const float *d_scale, const float *d_target, R eps, double *h_loss,
+                   R *d_gscale, double *h_gsig, double *h_galp, R *d_memb)
+{
+    DiffArgs<R> a{};
+    a.d = d;
+    a.sigma = sigma;
+    a.inv_s2 = (R)1 / (sigma * sigma);
+    a.alpha = alpha;
+    for (int k = 0; k < 27; k++) a.Dpq[k] = 1;
+    if (nadj == 27)
+        for (int k = 1; k < 27; k++) {
+            const int ax = Adjacency<27>::dx(k), ay = Adjacency<27>::dy(k), az = Adjacency<27>::dz(k);
+            a.Dpq[k] = (R)std::sqrt((R)(ax * ax + ay * ay + az * az));
+        }
+    const bool i16 = image_dtype == ROIFT_I16;
+    const R e = eps;
+    if (nadj == 7) {
+        if (i16) diff_run<7, int16_t, R>(c, a, d_image, d_seeds, n_seeds, d_label, lab_obj, niter, ckpt, d_scale,
+                                         d_target, e, h_loss, d_gscale, h_gsig, h_galp, d_memb);
+        else diff_run<7, int32_t, R>(c, a, d_image, d_seeds, n_seeds, d_label, lab_obj, niter, ckpt, d_scale,
+                                     d_target, e, h_loss, d_gscale, h_gsig, h_galp, d_memb);
+    } else {
+        if (i16) diff_run<27, int16_t, R>(c, a, d_image, d_seeds, n_seeds, d_label, lab_obj, niter, ckpt, d_scale,
+                                          d_target, e, h_loss, d_gscale, h_gsig, h_galp, d_memb);
+        else diff_run<27, int32_t, R>(c, a, d_image, d_seeds, n_seeds, d_label, lab_obj, niter, ckpt, d_scale,
+                                      d_target, e, h_loss, d_gscale, h_gsig, h_galp, d_memb);
+    }
+}
+
 }  // namespace
 
 int run_relax_diff(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype,
                    const int32_t *d_seeds, int n_seeds, const int32_t *d_label, int lab_obj, int niter, int ckpt,
-                   const float *d_scale, float sigma, float alpha, const float *d_target, float eps, double *h_loss,
-                   float *d_gscale, double *h_gsig, double *h_galp, float *d_memb)
+                   const float *d_scale, double sigma, double alpha, const float *d_target, double eps,
+                   double *h_loss, void *d_gscale, double *h_gsig, double *h_galp, void *d_memb, bool fp64)
 {
     ROIFT_REQUIRE(nadj == 7 || nadj == 27, ROIFT_EUNSUPPORTED, "adjacency must be Spheric(1.0) or Spheric(1.75)");
     ROIFT_REQUIRE(niter >= 0 && ckpt >= 1 && n_seeds >= 0, ROIFT_EINVAL, "bad iteration / checkpoint / seed count");
-    ROIFT_REQUIRE(sigma > 0.0f && eps > 0.0f && eps < 0.5f, ROIFT_EINVAL, "sigma must be > 0 and eps in (0, 0.5)");
+    ROIFT_REQUIRE(sigma > 0.0 && eps > 0.0 && eps < 0.5, ROIFT_EINVAL, "sigma must be > 0 and eps in (0, 0.5)");
     ROIFT_REQUIRE(d_scale && d_target && d_gscale, ROIFT_EINVAL, "scale, target and gradient buffers are required");
-    DiffArgs a{};
-    a.d = d;
-    a.sigma = sigma;
-    a.inv_s2 = 1.0f / (sigma * sigma);
-    a.alpha = alpha;
-    for (int k = 0; k < 27; k++) a.Dpq[k] = 1.0f;
-    if (nadj == 27)
-        for (int k = 1; k < 27; k++) {
-            const int ax = Adjacency<27>::dx(k), ay = Adjacency<27>::dy(k), az = Adjacency<27>::dz(k);
-            a.Dpq[k] = sqrtf((float)(ax * ax + ay * ay + az * az));
-        }
     ev_record(c, 12);
-    const bool i16 = image_dtype == ROIFT_I16;
-    if (nadj == 7) {
-        if (i16) diff_run<7, int16_t>(c, a, d_image, d_seeds, n_seeds, d_label, lab_obj, niter, ckpt, d_scale,
-                                      d_target, eps, h_loss, d_gscale, h_gsig, h_galp, d_memb);
-        else diff_run<7, int32_t>(c, a, d_image, d_seeds, n_seeds, d_label, lab_obj, niter, ckpt, d_scale, d_target,
-                                  eps, h_loss, d_gscale, h_gsig, h_galp, d_memb);
-    } else {
-        if (i16) diff_run<27, int16_t>(c, a, d_image, d_seeds, n_seeds, d_label, lab_obj, niter, ckpt, d_scale,
-                                       d_target, eps, h_loss, d_gscale, h_gsig, h_galp, d_memb);
-        else diff_run<27, int32_t>(c, a, d_image, d_seeds, n_seeds, d_label, lab_obj, niter, ckpt, d_scale,
-                                   d_target, eps, h_loss, d_gscale, h_gsig, h_galp, d_memb);
-    }
+    if (fp64)
+        diff_dispatch<double>(c, nadj, sigma, alpha, d, d_image, image_dtype, d_seeds, n_seeds, d_label, lab_obj,
+                              niter, ckpt, d_scale, d_target, eps, h_loss, (double *)d_gscale, h_gsig, h_galp,
+                              (double *)d_memb);
+    else   // the FP32 entry's float arguments, unchanged by the round trip through double
+        diff_dispatch<float>(c, nadj, (float)sigma, (float)alpha, d, d_image, image_dtype, d_seeds, n_seeds, d_label,
+                             lab_obj, niter, ckpt, d_scale, d_target, (float)eps, h_loss, (float *)d_gscale, h_gsig,
+                             h_galp, (float *)d_memb);
     ev_record(c, 13);
     c.ktimes[2] = ev_ms(c, 12, 13);
     c.ktimes[3] = 3 * niter;

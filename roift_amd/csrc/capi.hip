@@ -416,7 +416,26 @@ ROIFT_API int roift_relax_adjoint_dev(roift_ctx *ctx, const roift_problem *pb, c
         const int nadj = nadj_of(pb);
         ROIFT_REQUIRE(image_dtype == ROIFT_I16 || image_dtype == ROIFT_I32, ROIFT_EINVAL, "image dtype must be I16/I32");
         run_relax_diff(*ctx, d, nadj, d_image, image_dtype, d_seeds, n_seeds, d_label, lab_obj, niter, ckpt_every,
-                       d_scale, sigma, alpha, d_target, eps, h_loss, d_grad_scale, h_grad_sigma, h_grad_alpha, d_memb);
+                       d_scale, sigma, alpha, d_target, eps, h_loss, d_grad_scale, h_grad_sigma, h_grad_alpha, d_memb,
+                       false);
+        return ROIFT_OK;
+    });
+}
+
+ROIFT_API int roift_relax_adjoint64_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image,
+                                        int image_dtype, const int32_t *d_seeds, int n_seeds, const int32_t *d_label,
+                                        int lab_obj, int niter, int ckpt_every, const float *d_scale, double sigma,
+                                        double alpha, const float *d_target, double eps, double *h_loss,
+                                        double *d_grad_scale, double *h_grad_sigma, double *h_grad_alpha,
+                                        double *d_memb)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        const int nadj = nadj_of(pb);
+        ROIFT_REQUIRE(image_dtype == ROIFT_I16 || image_dtype == ROIFT_I32, ROIFT_EINVAL, "image dtype must be I16/I32");
+        run_relax_diff(*ctx, d, nadj, d_image, image_dtype, d_seeds, n_seeds, d_label, lab_obj, niter, ckpt_every,
+                       d_scale, sigma, alpha, d_target, eps, h_loss, d_grad_scale, h_grad_sigma, h_grad_alpha, d_memb,
+                       true);
         return ROIFT_OK;
     });
 }

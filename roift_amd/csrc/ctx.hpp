@@ -175,11 +175,12 @@ int run_batch(roift_ctx &c, int nvol, const int32_t *h_dims, float adj_r, float 
               int32_t *d_cost, bool use_graph);
 
 // Differentiable two-class relaxation and its checkpointed adjoint (learnable per-voxel
-// scale, RBF sigma and polarity alpha; BCE loss against d_target).
+// scale, RBF sigma and polarity alpha; BCE loss against d_target).  fp64: states,
+// gradients and memberships in double (d_gscale / d_memb are double *), else float.
 int run_relax_diff(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype,
                    const int32_t *d_seeds, int n_seeds, const int32_t *d_label, int lab_obj, int niter, int ckpt,
-                   const float *d_scale, float sigma, float alpha, const float *d_target, float eps, double *h_loss,
-                   float *d_gscale, double *h_gsig, double *h_galp, float *d_memb);
+                   const float *d_scale, double sigma, double alpha, const float *d_target, double eps,
+                   double *h_loss, void *d_gscale, double *h_gsig, double *h_galp, void *d_memb, bool fp64);
 
 // ORelax_1 (binary fallback): single channel f = label/Imax, hard polarity.  Weight source:
 // the image (W == Wx == nullptr), a per-voxel weight W, or per-axis Wx/Wy/Wz (6-adj only).

@@ -68,3 +68,55 @@ def test_checkpoint_interval_does_not_change_the_gradient(ctx):
         assert got[0] == pytest.approx(ref[0], rel=1e-12)
         assert np.array_equal(got[1].cpu().numpy(), ref[1].cpu().numpy())
         assert got[2] == pytest.approx(ref[2], rel=1e-9) and got[3] == pytest.approx(ref[3], rel=1e-9)
+
+
+def _gpu64(ctx, c, lab, scale, target, niter, sigma, alpha, adj, ckpt=10):
+    import torch
+    dev = torch.device("cuda")
+    return ctx.relax_adjoint(torch.from_numpy(c.image.astype(np.int16)).to(dev),
+                             torch.from_numpy(c.seeds.astype(np.int32)).to(dev),
+                             torch.from_numpy(lab.astype(np.int32)).to(dev), 1, niter,
+                             torch.from_numpy(scale).to(dev), sigma, alpha, torch.from_numpy(target).to(dev),
+                             ckpt_every=ckpt, adj_r=adj, eps=EPS, want_memb=True, fp64=True)
+
+
+@pytest.mark.parametrize("shape,seed,adj,niter", [((12, 14, 16), 31, 1.0, 25), ((10, 11, 12), 32, 1.75, 12)])
+def test_fp64_adjoint_meets_1e4_against_finite_differences(ctx, shape, seed, adj, niter):
+    """The FP64 form of the GPU adjoint: equal to the FP64 oracle's reverse-mode gradient
+    up to summation order, and within 1e-4 relative of FP64 central differences (h = 1e-4)
+    of the GPU's own FP64 loss, for sigma, alpha and single entries of the scale map."""
+    c, lab, scale, target = _setup(shape, seed, adj)
+    sigma, alpha, h = 30.0, 0.4, 1e-4
+    print(f"case shape={shape} adj={adj} niter={niter}", flush=True)
+    loss, gsc, gs, ga, f = _gpu64(ctx, c, lab, scale, target, niter, sigma, alpha, adj)
+    r = O.relax_diff(c.image, c.seeds, lab, 1, niter, scale.astype(np.float64), sigma, alpha,
+                     target.astype(np.float64), adj, eps=EPS)
+    g = gsc.cpu().numpy()
+    df = np.abs(f.cpu().numpy() - r["f"]).max()
+    print("fp64 vs oracle: f", df, "loss", abs(loss - r["loss"]) / abs(r["loss"]),
+          "sigma", abs(gs - r["g_sigma"]) / abs(r["g_sigma"]), "alpha", abs(ga - r["g_alpha"]) / abs(r["g_alpha"]),
+          "scale", np.abs(g - r["g_scale"]).max() / np.abs(r["g_scale"]).max(), flush=True)
+    # last-ulp differences of exp and of contracted products compound over the iterations
+    assert df <= 1e-12
+    assert abs(loss - r["loss"]) <= 1e-12 * abs(r["loss"])
+    assert abs(gs - r["g_sigma"]) <= 1e-10 * abs(r["g_sigma"])
+    assert abs(ga - r["g_alpha"]) <= 1e-10 * abs(r["g_alpha"])
+    assert np.abs(g - r["g_scale"]).max() <= 1e-10 * np.abs(r["g_scale"]).max()
+
+    def L(s=sigma, a=alpha, sc=scale):
+        return _gpu64(ctx, c, lab, sc, target, niter, s, a, adj)[0]
+
+    fd_s = (L(s=sigma + h) - L(s=sigma - h)) / (2 * h)
+    fd_a = (L(a=alpha + h) - L(a=alpha - h)) / (2 * h)
+    assert abs(gs - fd_s) <= 1e-4 * abs(fd_s)
+    assert abs(ga - fd_a) <= 1e-4 * abs(fd_a)
+    # the scale map is an FP32 input: perturb it by a step it can represent and compare
+    # with the FP64 gradient at the largest entries
+    for i in np.argsort(-np.abs(g.ravel()))[:3]:
+        hs = np.float32(1e-2)
+        up, dn = scale.copy(), scale.copy()
+        up.ravel()[i] += hs
+        dn.ravel()[i] -= hs
+        step = float(up.ravel()[i]) - float(dn.ravel()[i])
+        fd = (L(sc=up) - L(sc=dn)) / step
+        assert abs(g.ravel()[i] - fd) <= 1e-4 * abs(fd) + 1e-12
