@@ -112,7 +112,9 @@ def batched_bench(ctx, reps=1):
 def adjoint_bench(ctx, count=16, size=128, niter=50, ckpt=10):
     """16 volumes of 128^3: exact OIFT, then the differentiable relaxation (50 iterations)
     forward + checkpointed adjoint (every 10) against the Gaussian(sigma=2)-blurred true mask;
-    learnable per-voxel scale 1.0, RBF sigma = the image's global std, alpha 0.5."""
+    learnable per-voxel scale 1.0, alpha 0.5, RBF sigma = the image's global std: sigma is in
+    raw intensity units, and with intensities in [0, 4095] a sigma of 1.0 would make nearly
+    every FP32 weight exp(-d^2 / sigma^2) underflow to 0 (|d| >= 11)."""
     import torch
     import scipy.ndimage as nd
     from roift_amd import phantom as P
@@ -151,7 +153,7 @@ def adjoint_bench(ctx, count=16, size=128, niter=50, ckpt=10):
     torch.cuda.synchronize()
     dt64 = time.perf_counter() - t0
     return {"workload": f"{count} x {size}^3, differentiable relax {niter} iterations fwd + adjoint, "
-                        f"checkpoint every {ckpt}, BCE vs blurred truth",
+                        f"checkpoint every {ckpt}, BCE vs blurred truth, sigma = image std, alpha 0.5, scale 1",
             "voxels": nvox, "fwd_bwd_mvox_s": round(nvox / dt / 1e6, 2), "ms": round(dt * 1e3, 1),
             "mean_loss": round(float(np.mean(losses)), 6),
             "fp64_fwd_bwd_mvox_s": round(n64 * size ** 3 / dt64 / 1e6, 2)}
