@@ -13,7 +13,7 @@ from __future__ import annotations
 import json
 import os
 
-DEFAULT_CANDIDATES = [(64, 128), (128, 64), (128, 128), (128, 256), (192, 128), (256, 128)]
+DEFAULT_CANDIDATES = [(64, 1024), (96, 1024), (128, 128), (128, 1024), (192, 1024), (256, 128)]
 _cache: dict[str, tuple[int, int]] = {}
 
 
