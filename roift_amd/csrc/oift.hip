@@ -87,6 +87,9 @@ __device__ inline void st(uint32_t *p, uint32_t v)
 }
 
 struct State {
+    // 6-adjacency, 16-bit weights: every voxel's six arc weights of both classes in one
+    // 32-byte slot (u32 k-1 = class-1 weight << 16 | class-0 weight of arc k); null otherwise
+    const uint32_t *wpk;
     uint64_t *word;
     uint8_t *flags;
     uint64_t *run;    // per-class minimum rank of the current flood: class 0 low, class 1 high
