@@ -93,7 +93,7 @@ struct State {
     uint64_t *word;
     uint8_t *flags;
     uint64_t *run;    // per-class minimum rank of the current flood: class 0 low, class 1 high
-    uint32_t *tstamp, *qstamp, *estamp, *keystamp;
+    uint32_t *qstamp, *estamp, *keystamp;
     int32_t *keylab;
     uint8_t *keycls;
     uint32_t *rank_of_key;
@@ -126,7 +126,6 @@ __global__ void k_init(State s, long long n, long long nkeys)
         const bool frozen = s.label[p] != NIL;
         s.run[p] = frozen ? 0ull : INF64;   // frozen: no flood enters (see the rank-word notes)
         s.flags[p] = frozen ? FL_FROZEN : 0;
-        s.tstamp[p] = 0;
         s.qstamp[p] = 0;
         s.estamp[p] = 0;
 
@@ -245,7 +244,6 @@ int run_oift(roift_ctx &c, const Dims &d, int nadj, const void *d_wa, const void
     s.flags = buf<uint8_t>(c, "oift.flags", n);
     s.run = buf<uint64_t>(c, "oift.run", n);
 
-    s.tstamp = buf<uint32_t>(c, "oift.tstamp", n);
     s.qstamp = buf<uint32_t>(c, "oift.qstamp", n);
     s.estamp = buf<uint32_t>(c, "oift.estamp", n);
     const long long nkeys = 2 * n + 1024;
