@@ -241,3 +241,54 @@ def test_oift_brick_flood_mode_bit_exact(ctx, case):
         ctx.set_param("flood_mode", 0)
     o = O.oift_image(c.image, c.seeds, c.label_init(), adj, per)
     assert np.array_equal(lab, o["label"]) and np.array_equal(cost, o["cost"])
+
+
+SHAPES = [(1, 1024), (1, 1), (3, 2), (17, 1), (256, 1024)]
+
+
+@pytest.mark.parametrize("blocks,budget", SHAPES, ids=lambda v: str(v))
+def test_oift_launch_shapes_bit_exact(ctx, blocks, budget):
+    """The autotuner may pick any grid size and flood budget: one workgroup (it is always the
+    last to reach a barrier and runs every in-barrier step alone), grids below and above the
+    barrier's 16 arrival groups, and budget 1 (almost every push goes through the global spill
+    list), on a contested three-label volume, 26-adjacency, the scaled headline phantom
+    (bitmap-ranked steps) and tie-free weights."""
+    import torch
+    ctx.set_param("oift_blocks", blocks)
+    ctx.set_param("flood_budget", budget)
+    try:
+        c = P.small_case((18, 20, 22), 11, per=50.0, top=4095)
+        zs, ys, xs = c.image.shape
+        extra = np.array([3 + 4 * xs + 5 * xs * ys, 15 + 14 * xs + 12 * xs * ys], np.int64)
+        keep = ~np.isin(c.seeds, extra)
+        seeds = np.concatenate([extra[:1], c.seeds[keep], extra[1:]])
+        labels = np.concatenate([[2], c.seed_labels[keep], [3]]).astype(np.int32)
+        lab0 = np.full(c.image.size, -1, np.int32)
+        lab0[seeds] = labels
+        lab0 = lab0.reshape(c.image.shape)
+        pc = np.array([30, -30, -30, 30], np.float32)
+        lab, cost = run_gpu(ctx, c.image, seeds, lab0, 1.0, 0.0, pc)
+        o = O.oift_image(c.image, seeds, lab0, 1.0, per_class=pc)
+        assert np.array_equal(lab, o["label"]) and np.array_equal(cost, o["cost"]), "three labels"
+
+        c = P.small_case((13, 11, 17), 4, per=50.0, adj_r=1.75, top=4095)
+        lab, cost = run_gpu(ctx, c.image, c.seeds, c.label_init(), 1.75, 50.0)
+        o = O.oift_image(c.image, c.seeds, c.label_init(), 1.75, 50.0)
+        assert np.array_equal(lab, o["label"]) and np.array_equal(cost, o["cost"]), "26-adjacency"
+
+        c = P.headline_case(64)
+        lab, cost = run_gpu(ctx, c.image, c.seeds, c.label_init(), 1.0, 50.0)
+        o = O.oift_image(c.image, c.seeds, c.label_init(), 1.0, 50.0)
+        assert np.array_equal(lab, o["label"]) and np.array_equal(cost, o["cost"]), "headline 64^3"
+
+        c, wa, wb = tie_free_weights((10, 11, 12), 1.0, 0)
+        lab0 = c.label_init()
+        labt = dev(lab0)
+        cost = ctx.oift_edges(dev(wa.view(np.int32)), dev(wb.view(np.int32)), dev(c.seeds.astype(np.int32)), labt, 1.0)
+        torch.cuda.synchronize()
+        o = O.oift_edges(wa, wb, c.seeds, lab0, 1.0)
+        assert np.array_equal(labt.cpu().numpy(), o["label"]) and np.array_equal(cost.cpu().numpy(), o["cost"]), \
+            "tie-free"
+    finally:
+        ctx.set_param("oift_blocks", 0)
+        ctx.set_param("flood_budget", 0)
