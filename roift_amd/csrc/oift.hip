@@ -87,8 +87,9 @@ __device__ inline void st(uint32_t *p, uint32_t v)
 }
 
 struct State {
-    // 6-adjacency, 16-bit weights: every voxel's six arc weights of both classes in one
-    // 32-byte slot (u32 k-1 = class-1 weight << 16 | class-0 weight of arc k); null otherwise
+    // 16-bit weights: every voxel's arc weights of both classes in one slot (u32 k-1 =
+    // class-1 weight << 16 | class-0 weight of arc k; 32 bytes for 6-adjacency, 128 for
+    // 26-adjacency); null for 32-bit weights
     const uint32_t *wpk;
     uint64_t *word;
     uint8_t *flags;
