@@ -5,11 +5,14 @@ in HBM: oriented weights + exact OIFT + 50 FP32 relaxation iterations + final
 labels (roift_pipeline_dev), preceded by restoring the seed-only label map.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--no-cpu] [--no-batch] [--no-adjoint]
+                    [--no-config3]
 
 Prints one JSON line.  Its "blur" object times the CLI's preprocessing (two Gaussian-blur
 passes, one fused launch) on the same volume, outside the step.  Its "batched" object times the 64-volume batch (volumes of
 64-192 voxels a side) as 64 plain calls and as one batched call; its "adjoint" object
-times the differentiable relaxation's forward + checkpointed adjoint on 16 x 128^3.  The
+times the differentiable relaxation's forward + checkpointed adjoint on 16 x 128^3; its
+"config3" object times the 384x384x256 26-adjacency three-object volume (OIFT + 100
+Gaussian-RBF relaxation iterations).  The
 CPU baseline runs the reference's own
 OIFT_Multi + ORelax_1_Multi (compiled from its sources into oracle/_ref) on one
 core over a bounded sample, or the oracle restatement when that build is absent.
@@ -111,6 +114,34 @@ def batched_bench(ctx, reps=3):
             "oift_launch_shape": {"blocks": tune["blocks"], "budget": tune["budget"]}}
 
 
+def config3_bench(ctx, reps=2):
+    """384 x 384 x 256 voxels (z, y, x = 256, 384, 384), 26-adjacency, three objects: the exact
+    OIFT followed by the configuration's 100 Gaussian-RBF relaxation iterations (sigma 0.8);
+    best of `reps` after one warm-up."""
+    import torch
+    from roift_amd import phantom as P
+    c = P.multi_object_case()
+    n = int(c.image.size)
+    img = torch.from_numpy(c.image.astype(np.int16)).cuda()
+    seeds = torch.from_numpy(c.seeds.astype(np.int32)).cuda()
+    lab0 = torch.from_numpy(c.label_init()).cuda()
+    best, ko, kr = None, None, None
+    for r in range(reps + 1):
+        lab = lab0.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.oift_fmax(img, seeds, lab, 1.75, c.per, want_cost=False)
+        oift_k = ctx.kernel_timings()["oift_kernel_ms"]
+        ctx.relax_rbf(img, seeds, lab, 100, 0.8, 1.75, want_memb=False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if r and (best is None or dt < best):
+            best, ko, kr = dt, oift_k, ctx.timings()["relax_ms"]
+    return {"workload": "3 objects 384x384x256 (z 256), 26-adj, oift_fmax + 100 Gaussian-RBF relax iterations "
+                        "(sigma 0.8)", "voxels": n, "mvox_s": round(n / best / 1e6, 2), "ms": round(best * 1e3, 1),
+            "oift_kernel_ms": round(ko, 1), "relax_ms": round(kr, 1)}
+
+
 def adjoint_bench(ctx, count=16, size=128, niter=50, ckpt=10):
     """16 volumes of 128^3: exact OIFT, then the differentiable relaxation (50 iterations)
     forward + checkpointed adjoint (every 10) against the Gaussian(sigma=2)-blurred true mask;
@@ -171,6 +202,7 @@ def main():
     ap.add_argument("--probe-copy", action="store_true", help="also run a 1 GiB device copy (profiler scaling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-volume batched-vs-plain measurement")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 16 x 128^3 relaxation-adjoint measurement")
+    ap.add_argument("--no-config3", action="store_true", help="skip the 384x384x256 26-adjacency measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -311,6 +343,8 @@ def main():
         res["batched"] = batched_bench(ctx)
     if not args.no_adjoint:
         res["adjoint"] = adjoint_bench(ctx)
+    if not args.no_config3:
+        res["config3"] = config3_bench(ctx)
     if args.probe_copy:
         # a plain copy of known size, for scaling the profiler's FETCH_SIZE / WRITE_SIZE
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
