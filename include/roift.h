@@ -81,7 +81,8 @@ ROIFT_API int roift_oift_profile(const roift_ctx *ctx, double *out, int n);
 /* Tuning knobs (the autotuner in roift_amd/tune.py sets them):
    ROIFT_PARAM_OIFT_BLOCKS    workgroups of the persistent OIFT kernel (0 = half the CUs)
    ROIFT_PARAM_FLOOD_BUDGET   LDS batches a workgroup relaxes per flood round before spilling (0 = 1024)
-   ROIFT_PARAM_RELAX_DEPTH    relaxation iterations fused per pass (temporal-block depth)
+   ROIFT_PARAM_RELAX_DEPTH    reserved: accepted and ignored (each relaxation iteration is one
+                              launch; the kernels are bound by arithmetic, not by HBM)
    ROIFT_PARAM_FLOOD_MODE     OIFT flood unit: 0 = single voxels (default), 1 = 8x8x8
                               bricks relaxed in LDS (6-adjacency only; slower on thin fronts)
    ROIFT_PARAM_RELAX_TILE     two-class 6-adjacency relaxation tile: rows (1..8) | planes << 8

@@ -18,7 +18,7 @@ struct roift_ctx {
                                     // flood items [162], touched slots [163], batches sum [164] / max per workgroup [165]
     int tune_blocks = 0;            // OIFT persistent grid size (0: half the CUs)
     int tune_budget = 0;            // OIFT LDS batches per workgroup per flood round (0: 1024)
-    int tune_relax_depth = 0;       // relaxation temporal-block depth (0: default)
+    int tune_relax_depth = 0;       // reserved (ROIFT_PARAM_RELAX_DEPTH is accepted and ignored)
     int tune_flood_mode = 0;        // OIFT flood: 0 = single voxels, 1 = 8x8x8 bricks (6-adjacency)
     int tune_relax_tile = 0;        // z-marching relaxation tile: rows | planes << 8 | xcd << 16 (0: default)
     int tune_blur_planes = 0;       // Gaussian blur: planes per workgroup (0: automatic)
