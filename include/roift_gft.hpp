@@ -15,7 +15,8 @@
 // gft_adjrel3.h:10-23) because callers allocate these records with gft's own
 // constructors; only the fields this path reads are used (data, sizes, d, n).
 // A program that includes the reference's gft headers does not include this
-// file: it links -lroift_gft ahead of -lgft.
+// file: it links -lroift_gft ahead of -lgft, so that these definitions are the ones the
+// linker binds.
 #ifndef ROIFT_GFT_HPP
 #define ROIFT_GFT_HPP
 
