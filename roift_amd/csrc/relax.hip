@@ -329,6 +329,15 @@ __global__ __launch_bounds__(MX *MY_MAX) void k_relax2_march(const int16_t *__re
     }
     int rc = LI(p);
     float fc = LF(p);
+    // The in-plane neighbours are loaded unconditionally, so that all of a voxel's loads are
+    // in flight together (loads under the bounds tests would each wait for their own data):
+    // an absent neighbour reads the voxel itself, which is in the volume, and its weight is
+    // zeroed.  Adding a +0 weight and a +0 product leaves both sums bit-identical, since the
+    // weights and memberships are non-negative.
+    const uint32_t kym = y > 0 ? ~0u : 0u, kxm = x > 0 ? ~0u : 0u, kxp = x < xs - 1 ? ~0u : 0u,
+                   kyp = y < ys - 1 ? ~0u : 0u;
+    const uint32_t dym = kym & uxs, dxm = kxm & 1u, dxp = kxp & 1u, dyp = kyp & uxs;
+    auto keep = [](float w, uint32_t k) { return __uint_as_float(__float_as_uint(w) & k); };
     for (int z = z0; z < z1; z++, p += xy) {
         const bool hp = z + 1 < zs;
         int rn = 0;
@@ -337,6 +346,8 @@ __global__ __launch_bounds__(MX *MY_MAX) void k_relax2_march(const int16_t *__re
             rn = LI(p + xy);
             fn = LF(p + xy);
         }
+        const int rym = LI(p - dym), rxm = LI(p - dxm), rxp = LI(p + dxp), ryp = LI(p + dyp);
+        const float fym = LF(p - dym), fxm = LF(p - dxm), fxp = LF(p + dxp), fyp = LF(p + dyp);
         const float Ic = (float)(rc & 0x7FFF);
         float v = fc;
         if (!(rc & 0x8000)) {
@@ -347,30 +358,18 @@ __global__ __launch_bounds__(MX *MY_MAX) void k_relax2_march(const int16_t *__re
                 sw += w;
                 acc = fmaf(w, fm, acc);
             }
-            if (y > 0) {
-                const float fq = LF(p - uxs);
-                w = relax_w8(Ic - (float)(LI(p - uxs) & 0x7FFF), fq, a);
-                sw += w;
-                acc = fmaf(w, fq, acc);
-            }
-            if (x > 0) {
-                const float fq = LF(p - 1);
-                w = relax_w8(Ic - (float)(LI(p - 1) & 0x7FFF), fq, a);
-                sw += w;
-                acc = fmaf(w, fq, acc);
-            }
-            if (x < xs - 1) {
-                const float fq = LF(p + 1);
-                w = relax_w8(Ic - (float)(LI(p + 1) & 0x7FFF), fq, a);
-                sw += w;
-                acc = fmaf(w, fq, acc);
-            }
-            if (y < ys - 1) {
-                const float fq = LF(p + uxs);
-                w = relax_w8(Ic - (float)(LI(p + uxs) & 0x7FFF), fq, a);
-                sw += w;
-                acc = fmaf(w, fq, acc);
-            }
+            w = keep(relax_w8(Ic - (float)(rym & 0x7FFF), fym, a), kym);
+            sw += w;
+            acc = fmaf(w, fym, acc);
+            w = keep(relax_w8(Ic - (float)(rxm & 0x7FFF), fxm, a), kxm);
+            sw += w;
+            acc = fmaf(w, fxm, acc);
+            w = keep(relax_w8(Ic - (float)(rxp & 0x7FFF), fxp, a), kxp);
+            sw += w;
+            acc = fmaf(w, fxp, acc);
+            w = keep(relax_w8(Ic - (float)(ryp & 0x7FFF), fyp, a), kyp);
+            sw += w;
+            acc = fmaf(w, fyp, acc);
             if (hp) {
                 const float In = (float)(rn & 0x7FFF);
                 w = relax_w8(Ic - In, fn, a);
