@@ -12,7 +12,8 @@ passes, one fused launch) on the same volume, outside the step.  Its "batched" o
 64-192 voxels a side) as 64 plain calls and as one batched call; its "adjoint" object
 times the differentiable relaxation's forward + checkpointed adjoint on 16 x 128^3; its
 "config3" object times the 384x384x256 26-adjacency three-object volume (OIFT + 100
-Gaussian-RBF relaxation iterations).  The
+Gaussian-RBF relaxation iterations); "hbm_copy_measured" is a timed 1 GiB device copy, the
+measured bandwidth beside the 8 TB/s specification the rooflines use.  The
 CPU baseline runs the reference's own
 OIFT_Multi + ORelax_1_Multi (compiled from its sources into oracle/_ref) on one
 core over a bounded sample, or the oracle restatement when that build is absent.
@@ -37,6 +38,28 @@ ALG = dict(weights=2 + 3 * 2 * 2, oift=12 + 4 + 4 + 4, relax_iter=2 + 4 + 4, lab
 # workload, scaled by the same counters on a 1 GiB copy (profiles/pmc_traffic_headline256.md).
 TRAFFIC = {"k_oift_persistent": 1488.35, "k_relax2_march": 10.45, "k_weights": 16.13, "k_blur": 4.71,
            "k_sobel": 7.34}
+
+
+def copy_peak(dev, nbytes=1 << 30, reps=5):
+    """Measured HBM bandwidth of a plain device copy (torch's copy kernel, 1 GiB read + 1 GiB
+    written), best of `reps`, beside the 8 TB/s specification the rooflines are priced on."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    del src, dst
+    gbs = 2 * nbytes / (best / 1e3) / 1e9
+    return {"GB/s": round(gbs, 1), "bytes_moved": 2 * nbytes, "ms": round(best, 4),
+            "frac_of_spec": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def cpu_baseline(size=128, runs=3):
@@ -345,6 +368,7 @@ def main():
         res["adjoint"] = adjoint_bench(ctx)
     if not args.no_config3:
         res["config3"] = config3_bench(ctx)
+    res["hbm_copy_measured"] = copy_peak(dev)
     if args.probe_copy:
         # a plain copy of known size, for scaling the profiler's FETCH_SIZE / WRITE_SIZE
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
