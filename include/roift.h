@@ -79,7 +79,8 @@ ROIFT_API int roift_kernel_timings(const roift_ctx *ctx, double *out, int n);
 ROIFT_API int roift_oift_profile(const roift_ctx *ctx, double *out, int n);
 
 /* Tuning knobs (the autotuner in roift_amd/tune.py sets them):
-   ROIFT_PARAM_OIFT_BLOCKS    workgroups of the persistent OIFT kernel (0 = half the CUs)
+   ROIFT_PARAM_OIFT_BLOCKS    workgroups of the persistent OIFT kernel (0 = 0.27 * cbrt(voxels *
+                              (adjacency - 1)), a multiple of 8, at least 32, at most the co-resident limit)
    ROIFT_PARAM_FLOOD_BUDGET   LDS batches a workgroup relaxes per flood round before spilling (0 = 1024)
    ROIFT_PARAM_RELAX_DEPTH    reserved: accepted and ignored (each relaxation iteration is one
                               launch; the kernels are bound by arithmetic, not by HBM)

@@ -16,7 +16,7 @@ struct roift_ctx {
     double timings[6] = {0, 0, 0, 0, 0, 0};
     double prof[166] = {};          // OIFT phase ms [0..8], counts [9..17], ms by log2(entries) [18..161],
                                     // flood items [162], touched slots [163], batches sum [164] / max per workgroup [165]
-    int tune_blocks = 0;            // OIFT persistent grid size (0: half the CUs)
+    int tune_blocks = 0;            // OIFT persistent grid size (0: sized from the arc count)
     int tune_budget = 0;            // OIFT LDS batches per workgroup per flood round (0: 1024)
     int tune_relax_depth = 0;       // reserved (ROIFT_PARAM_RELAX_DEPTH is accepted and ignored)
     int tune_flood_mode = 0;        // OIFT flood: 0 = single voxels, 1 = 8x8x8 bricks (6-adjacency)

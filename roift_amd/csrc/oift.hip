@@ -33,6 +33,8 @@
 //
 // Bound: latency of the barrier-separated phases; algorithmic bytes per voxel:
 // weights read once (2E values), label in/out, cost out.
+#include <cmath>
+
 #include "ctx.hpp"
 
 namespace roift {
