@@ -13,7 +13,8 @@ from __future__ import annotations
 import json
 import os
 
-DEFAULT_CANDIDATES = [(64, 1024), (96, 1024), (128, 128), (128, 1024), (192, 1024), (256, 128)]
+# blocks 0 = the library's default grid, sized from the workload's arc count
+DEFAULT_CANDIDATES = [(0, 1024), (64, 1024), (96, 1024), (128, 128), (128, 1024), (192, 1024), (256, 128)]
 _cache: dict[str, tuple[int, int]] = {}
 
 
@@ -38,7 +39,7 @@ def autotune_oift(ctx, key: str, run, candidates=None, reps: int = 1) -> dict:
     run() must launch the OIFT on ctx (alone or inside a pipeline).  Each candidate is
     timed by the OIFT kernel's device-event time (ctx.kernel_timings()), best of `reps`
     runs after one warm-up.  Returns {"blocks", "budget", "ms", "trials"}; the winner is
-    left set on ctx.
+    left set on ctx (blocks 0: the library's own size-based default).
     """
     _load_file()
     if key in _cache:
