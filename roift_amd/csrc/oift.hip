@@ -42,6 +42,11 @@ namespace {
 
 enum : uint8_t { FL_FROZEN = 1, FL_SETTLED = 2 };
 
+// Counter slots, laid out by 128-byte cache line (32 words).  The words that waiting workgroups
+// poll (the barrier generation and the abort flag), the flood's work cursors, the barrier's
+// top counter and each group counter sit on lines of their own: polls and atomics on one line
+// are served one after another, and the workgroups that still work in a phase would otherwise
+// queue behind the ones spinning at its end.  The first K_HOST words are copied to the host.
 enum Ctr : int {
     K_LEVEL = 0,    // [2] by level parity
     K_F = 2,        // [2] frontier list counts
@@ -50,7 +55,7 @@ enum Ctr : int {
     K_NS = 7,       // voxels settled in the step
     K_T = 8,        // [2] touched, by iteration parity
     K_WC = 10,      // [3] worklist counts, by round mod 3
-    K_CUR = 13,     // [3] worklist cursors, by round mod 3
+    K_KW0 = 13,     // [2] first bitmap word of the step's keys, by step parity
     K_RSAFE = 16,   // [2] by iteration parity
     K_LEFT = 18,    // [2] by iteration parity
     K_ERR = 20,
@@ -62,13 +67,15 @@ enum Ctr : int {
     K_NROUNDS = 26,
     K_FIN = 27,     // which frontier list is live when the kernel starts
     K_CONT = 28,    // [2] "two classes met" flag, by iteration parity
-    K_BAR_ARRIVE = 30,
-    K_BAR_GEN = 31,
-    K_ABORT = 32,
-    K_KW0 = 34,     // [2] first bitmap word of the step's keys, by step parity
-    K_KW1 = 36,     // [2] last bitmap word
-    K_BAR_GROUP = 40,   // [BAR_GROUPS] first-level barrier counters
-    K_COUNT = 64
+    K_KW1 = 30,     // [2] last bitmap word
+    K_ABORT = 32,   // line 1: polled by waiting workgroups
+    K_BAR_GEN = 33,
+    K_HOST = 64,    // words mirrored on the host
+    K_CUR = 64,     // line 2: [3] worklist cursors, by round mod 3
+    K_BAR_ARRIVE = 96,    // line 3
+    K_BAR_GROUP = 128,    // lines 4..: [BAR_GROUPS] first-level barrier counters, K_LINE apart
+    K_LINE = 32,
+    K_COUNT = 128 + 16 * 32
 };
 
 constexpr int BLOCK = 512;
@@ -76,6 +83,7 @@ constexpr int SORT_MAX = 16384;     // LDS words: bitonic sort capacity and floo
 constexpr int QCAP = SORT_MAX;
 constexpr int QUAD_MAX = 64;        // steps with at most this many entries rank keys in one wavefront
 constexpr uint32_t BAR_GROUPS = 16;
+static_assert(K_BAR_GROUP + (int)BAR_GROUPS * K_LINE <= K_COUNT, "barrier group lines fit the counters");
 constexpr int MAX_BLOCKS = 1024;   // persistent grid cap (bounds the touched list's chunk slack)
 
 __device__ inline uint32_t ld(const uint32_t *p)
@@ -137,8 +145,8 @@ __global__ void k_init(State s, long long n, long long nkeys)
     for (long long k = gtid(); k < nkeys; k += gstride()) s.keystamp[k] = 0;
     for (long long w = gtid(); w < nkeys / 64 + 1; w += gstride()) s.bm[w] = 0;
     if (gtid() < 256) s.prof[gtid()] = 0;
-    if (gtid() < K_COUNT) {  // one writer per counter slot
-        const int i = (int)gtid();
+    for (long long k = gtid(); k < K_COUNT; k += gstride()) {   // one writer per counter slot
+        const int i = (int)k;
         const bool inf = i == K_LEVEL || i == K_LEVEL + 1 || i == K_RSAFE || i == K_RSAFE + 1 || i == K_KW0 ||
                          i == K_KW0 + 1;
         s.ctr[i] = inf ? INF32 : 0u;
