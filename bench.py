@@ -36,7 +36,7 @@ ALG = dict(weights=2 + 3 * 2 * 2, oift=12 + 4 + 4 + 4, relax_iter=2 + 4 + 4, lab
            sobel=2 + 4)   # Sobel: read the int16 image, write the int32 gradient
 # HBM bytes per voxel per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
 # workload, scaled by the same counters on a 1 GiB copy (profiles/pmc_traffic_headline256.md).
-TRAFFIC = {"k_oift_persistent": 1498.41, "k_relax2_march": 10.51, "k_weights": 16.13, "k_blur": 4.71,
+TRAFFIC = {"k_oift_persistent": 1489.83, "k_relax2_march": 10.51, "k_weights": 16.13, "k_blur": 4.71,
            "k_sobel": 7.34}
 
 
