@@ -38,6 +38,15 @@ ALG = dict(weights=2 + 3 * 2 * 2, oift=12 + 4 + 4 + 4, relax_iter=2 + 4 + 4, lab
 # workload, scaled by the same counters on a 1 GiB copy (profiles/pmc_traffic_headline256.md).
 TRAFFIC = {"k_oift_persistent": 1489.83, "k_relax2_march": 10.51, "k_weights": 16.13, "k_blur": 4.71,
            "k_sobel": 7.34}
+# What limits each kernel, from its profiles (rocprofv3 counters and A/B timings):
+LIMITED_BY = {
+    "k_oift_persistent": "latency: ~2,100 grid barriers and the flood's dependent atomic round trips; "
+                         "HBM traffic is a by-product of the L2 write-back/invalidate at each barrier",
+    "k_relax2_march": "VALU: ~105 instructions per voxel (the eighth power, the division, six neighbours)",
+    "k_weights": "HBM",
+    "k_blur": "VALU: the reference's FP32 tap order, ~50 instructions per output voxel",
+    "k_sobel": "L1/TA: 54 taps per voxel shared 4-wide",
+}
 
 
 def copy_peak(dev, nbytes=1 << 30, reps=5):
@@ -62,27 +71,58 @@ def copy_peak(dev, nbytes=1 << 30, reps=5):
             "frac_of_spec": round(gbs / HBM_PEAK_GBS, 4)}
 
 
-def cpu_baseline(size=128, runs=3):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(size=256, runs=1, second=(128, 3)):
+    """The reference's own OIFT_Multi + ORelax_1_Multi (oracle/_ref, compiled -O3 from its
+    sources; the oracle restatement where that build is absent) on the headline phantom, on
+    one host core: the process is pinned to one core with sched_setaffinity for the timed
+    calls.  `runs` runs at `size` (the median is the baseline) and, as a second figure,
+    second[1] runs at second[0]^3."""
     import oracle as O
     from roift_amd import phantom as P
-    c = P.headline_case(size)
-    lab0 = c.label_init()
-    kind = "reference" if O.ref_lib() is not None else "port"
-    times = []
-    for _ in range(runs):
-        if kind == "reference":
-            _, t1, t2 = O.ref_pipeline(c.image, c.seeds, lab0, c.niter, 1.0, c.per)
-            times.append(t1 + t2)
-        else:
-            t0 = time.perf_counter()
-            o = O.oift_image(c.image, c.seeds, lab0, 1.0, c.per)
-            O.relax(c.image, c.seeds, o["label"], c.niter, 1.0, c.per)
-            times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    return {"value": round(c.image.size / t / 1e6, 4), "unit": "Mvoxels/s", "cores": 1, "kind": kind,
-            "sample": f"{size}^3 headline phantom (same generator), OIFT_Multi + 50-iteration ORelax_1_Multi, "
-                      f"median of {runs} runs of {t:.2f} s, 1 of {os.cpu_count()} host cores, "
-                      f"{'reference gft sources -O3' if kind == 'reference' else 'oracle restatement -O3'}"}
+    kind = "reference" if O.ref_available() else "port"
+    allowed = sorted(os.sched_getaffinity(0))
+    core = allowed[-1]
+    os.sched_setaffinity(0, {core})
+
+    def timed(n, k):
+        c = P.headline_case(n)
+        lab0 = c.label_init()
+        ts = []
+        for _ in range(k):
+            if kind == "reference":
+                _, t1, t2 = O.ref_pipeline(c.image, c.seeds, lab0, c.niter, 1.0, c.per)
+                ts.append(t1 + t2)
+            else:
+                t0 = time.perf_counter()
+                o = O.oift_image(c.image, c.seeds, lab0, 1.0, c.per)
+                O.relax(c.image, c.seeds, o["label"], c.niter, 1.0, c.per)
+                ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), ts, c.image.size
+
+    try:
+        t, ts, nvox = timed(size, runs)
+        t2, ts2, nvox2 = timed(*second)
+    finally:
+        os.sched_setaffinity(0, set(allowed))
+    impl = "reference gft sources -O3" if kind == "reference" else "oracle restatement -O3"
+    return {"value": round(nvox / t / 1e6, 4), "unit": "Mvoxels/s", "cores": 1, "kind": kind,
+            "sample": f"{size}^3 headline phantom (the bench workload, same generator), OIFT_Multi + "
+                      f"{50}-iteration ORelax_1_Multi, {runs} run(s) of {', '.join(f'{x:.2f}' for x in ts)} s "
+                      f"(median), pinned to host core {core} of {len(allowed)} allowed "
+                      f"({os.cpu_count()} online), CPU {_cpu_model()}, {impl}",
+            "second_figure": {"size": f"{second[0]}^3", "runs": second[1], "median_s": round(t2, 3),
+                              "Mvoxels/s": round(nvox2 / t2 / 1e6, 4)}}
 
 
 def batched_bench(ctx, reps=3):
@@ -126,7 +166,7 @@ def batched_bench(ctx, reps=3):
     ctx.set_param("oift_blocks", 0)
     ctx.set_param("flood_budget", 0)
     t_plain = timed(plain)
-    tune = autotune_oift(ctx, "batch64x[64,192]^3", batched, reps=2)
+    tune = autotune_oift(ctx, "batch64x[64,192]^3", batched, reps=3)
     t_batch = timed(batched)
     ctx.set_param("oift_blocks", 0)
     ctx.set_param("flood_budget", 0)
@@ -275,10 +315,12 @@ def main():
     # Rooflines from device events recorded on the library's stream around each kernel:
     # algorithmic bytes per launch / average launch duration.
     def roof(name, alg_per_voxel, launch_ms):
+        # "bound" names the roof the kernel is priced against (HBM: nothing here is a
+        # contraction); "limited_by" is what the profiles show actually limits it
         gbs = alg_per_voxel * n / (launch_ms / 1e3) / 1e9
-        out = {"kernel": name, "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-               "frac": round(gbs / HBM_PEAK_GBS, 5), "launch_ms": round(launch_ms, 4),
-               "alg_bytes_per_launch": alg_per_voxel * n}
+        out = {"kernel": name, "bound": "hbm", "limited_by": LIMITED_BY.get(name), "achieved": round(gbs, 2),
+               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5),
+               "launch_ms": round(launch_ms, 4), "alg_bytes_per_launch": alg_per_voxel * n}
         tr = TRAFFIC.get(name)
         out["traffic"] = int(tr * n) if tr is not None else None
         return out
@@ -376,9 +418,11 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline()
+        res["cpu_baseline"] = cpu_baseline(size=args.size)
         res["speedup_vs_cpu"] = round(res["value"] / res["cpu_baseline"]["value"], 1)
-    print(json.dumps(res))
+    torch.cuda.synchronize()
+    ctx.close()   # release the device memory before the interpreter (and a profiler) shut down
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

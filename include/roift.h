@@ -49,6 +49,13 @@ typedef struct {
     float per;                 /* polarity x100 when use_per_class == 0 */
     const float *h_per_class;  /* length max_label+1 when use_per_class == 1 */
     int max_label;
+    float dx, dy, dz;          /* voxel spacing (gft's scn->dx/dy/dz); 0 reads as 1.0, so a
+                                  zero-initialised tail means unit spacing; any other finite
+                                  value is used as gft uses it.  Read by the
+                                  relaxations' Dpq = sqrtf(sum (d_i spacing_i)^2) / min spacing
+                                  (gft_ift.cpp:8723-8731) and the Gaussian blur's taps
+                                  (Kernel3::SphericalGaussian(scn, ...), gft_filtering3.cpp:106-126);
+                                  the OIFT's arc weights do not depend on it. */
 } roift_problem;
 
 /* ---- context ---------------------------------------------------------------- */
@@ -111,6 +118,15 @@ ROIFT_API int roift_set_param(roift_ctx *ctx, int param, int value);
 ROIFT_API int roift_weights_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
                                 void *d_wa, void *d_wb, int weight_dtype, int32_t *h_maxI, int32_t *h_minI);
 
+/* The same layout for the per-voxel weight forms of the OIFT: base W_p + W_q (d_W, int32 N)
+ * or, on Spheric(1.0) only, W_t[p] + W_t[q] with t the arc's axis (d_Wx, d_Wy, d_Wz, int32 N
+ * each), times (1 +- per/100) in double by the order of the image values I_p, I_q (unchanged
+ * when equal), truncated (gft_ift.cpp:1142-1152, 1244-1254).  Give d_W or all three axes,
+ * the others NULL.  Weights must be >= 0; the image may hold any int values.          */
+ROIFT_API int roift_weights_w_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                  const int32_t *d_W, const int32_t *d_Wx, const int32_t *d_Wy, const int32_t *d_Wz,
+                                  void *d_wa, void *d_wb, int weight_dtype);
+
 /* ---- OIFT under f_max ------------------------------------------------------------ */
 /* Drop-in for gft::ift::OIFT_Multi / OIFT_Multi_PerClass (gft_ift.h:108-122,
  * gft_ift.cpp:911-1083).  d_label is in/out exactly as the reference's label
@@ -119,6 +135,17 @@ ROIFT_API int roift_weights_dev(roift_ctx *ctx, const roift_problem *pb, const v
  * C(q) = max(C(pred q), w(pred q -> q)) (INT32_MAX for voxels never reached).  */
 ROIFT_API int roift_oift_fmax_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
                                   const int32_t *d_seeds, int n_seeds, int32_t *d_label, int32_t *d_cost);
+
+/* Drop-in for the per-voxel weight forms gft::ift::OIFT(W, A, scn, per, S, label)
+ * (gft_ift.h:125-130, gft_ift.cpp:1086-1167) and OIFT(Wx, Wy, Wz, scn, per, S, label)
+ * (gft_ift.h:133-139, gft_ift.cpp:1170-1271): arc weights as roift_weights_w_dev builds
+ * them, label > 0 -> +per, label 0 -> -per, the same Prim order and FIFO ties as
+ * roift_oift_fmax_dev, so labels and the f_max cost map are the reference's bit for bit.
+ * pb->use_per_class must be 0.  (gft::ift::OIFT(A, scn, per, S, label), gft_ift.h:101-106,
+ * is OIFT_Multi for labels >= 0: roift_oift_fmax_dev.)                                    */
+ROIFT_API int roift_oift_w_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                               const int32_t *d_W, const int32_t *d_Wx, const int32_t *d_Wy, const int32_t *d_Wz,
+                               const int32_t *d_seeds, int n_seeds, int32_t *d_label, int32_t *d_cost);
 
 /* Same forest on caller-supplied oriented edge weights (layout as
  * roift_weights_dev writes it, ROIFT_U16 or ROIFT_U32).  Labels for which
@@ -319,6 +346,9 @@ ROIFT_API int roift_cli_main(int argc, char **argv);
 /* Host-pointer forms (copy in, run, copy out).  h_image is int32 as gft's Scene32. */
 ROIFT_API int roift_oift_fmax(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image,
                               const int32_t *h_seeds, int n_seeds, int32_t *h_label, int32_t *h_cost);
+ROIFT_API int roift_oift_w(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image, const int32_t *h_W,
+                           const int32_t *h_Wx, const int32_t *h_Wy, const int32_t *h_Wz, const int32_t *h_seeds,
+                           int n_seeds, int32_t *h_label, int32_t *h_cost);
 ROIFT_API int roift_relax(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image,
                           const int32_t *h_seeds, int n_seeds, int32_t *h_label, int niter, float *h_memb);
 ROIFT_API int roift_relax_binary(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image, const int32_t *h_W,
@@ -328,13 +358,28 @@ ROIFT_API int roift_gaussian_blur(roift_ctx *ctx, const roift_problem *pb, const
                                   int passes);
 
 /* ---- legacy reference C-ABI ---------------------------------------------------------- */
-/* Same symbol and signature as the reference's GPU entry point
- * (gpu/oift_fmax_gpu.cu:123-147, declared by its caller at
- * gpu/oiftrelax_fmax_gpu.cpp:21-25).  Computes OIFT_Multi_PerClass with the
- * given per-class polarities, exactly; h_labels is in/out.  Returns the
- * number of level steps, or -1 on error.                                        */
+/* Same symbols and signatures as the reference's two GPU entry points.  Both compute
+ * OIFT_Multi_PerClass with the given per-class polarities exactly (6-adjacency), under the
+ * reference's input contract: every voxel that is not a listed seed starts unlabelled,
+ * whatever h_labels holds; a seed's label is h_labels[seed] with negative labels read as 0;
+ * seeds outside [0, n) are skipped and a repeated seed counts once (its first position is
+ * its FIFO position); on return every voxel holds its label, 0 where no seed reached it.
+ * Return: the number of level steps (the reference returns its wave count), or -1 on
+ * error (message on stderr).  Per-class tables with more than two polarities (other than
+ * +P / -P) are outside this library and return -1.
+ *
+ * oift_fmax_gpu_run: gpu/oift_fmax_gpu.cu:123-147, declared by its caller at
+ * gpu/oiftrelax_fmax_gpu.cpp:21-25.  With no valid seed every label becomes 0 (the
+ * reference's untouched (COST_INF, 0) words) and it returns 0.                          */
 ROIFT_API int oift_fmax_gpu_run(const int *h_image, int *h_labels, int n, int xsize, int ysize, int zsize,
                                 const float *h_per_class, int max_label, const int *h_seeds, int n_seeds);
+/* oift_gpu_run: gpu/oift_gpu.cu:149-158, declared by its caller at gpu/oiftrelax_gpu.cpp:22-28
+ * and called at :184-189.  cost_mode 0 = f_max (the path this library implements); any
+ * other mode (1 = additive) returns -1.  Seed labels above max_label are read as 0
+ * (oift_gpu.cu:202); with no valid seed h_labels is left as it is and it returns 0
+ * (oift_gpu.cu:207-210).                                                                */
+ROIFT_API int oift_gpu_run(int cost_mode, const int *h_image, int *h_labels, int n, int xsize, int ysize,
+                           int zsize, const float *h_per_class, int max_label, const int *h_seeds, int n_seeds);
 
 #ifdef __cplusplus
 }

@@ -1,7 +1,9 @@
-// roift_gft.hpp -- link-level drop-in for the four gft entry points on the hot path.
+// roift_gft.hpp -- link-level drop-in for the gft entry points on the hot path.
 //
 // libroift_gft.so defines, with the same qualified names, signatures and types
 // as the reference's gft library, the functions its callers use on this path:
+//   gft::ift::OIFT (A, scn, ...), (W, A, scn, ...) and (Wx, Wy, Wz, scn, ...)
+//                                     (gft_ift.h:101-106, 125-139)
 //   gft::ift::OIFT_Multi              (gft_ift.h:108-112)
 //   gft::ift::OIFT_Multi_PerClass     (gft_ift.h:117-122)
 //   gft::ift::ORelax_1_Multi          (gft_ift.h:451-456)
@@ -54,6 +56,9 @@ typedef AdjRel3::sAdjRel3 sAdjRel3;
 typedef Scene32::sScene32 sScene32;
 
 namespace ift {
+void OIFT(sAdjRel3 *A, sScene32 *scn, float per, int *S, sScene32 *label);
+void OIFT(sScene32 *W, sAdjRel3 *A, sScene32 *scn, float per, int *S, sScene32 *label);
+void OIFT(sScene32 *Wx, sScene32 *Wy, sScene32 *Wz, sScene32 *scn, float per, int *S, sScene32 *label);
 void OIFT_Multi(sAdjRel3 *A, sScene32 *scn, float per, int *S, sScene32 *label);
 void OIFT_Multi_PerClass(sAdjRel3 *A, sScene32 *scn, const float *per_class, int max_label, int *S,
                          sScene32 *label);

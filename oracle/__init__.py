@@ -80,8 +80,17 @@ def lib():
                                          f64p, C.c_double, C.c_double, f64p, C.c_double, f64p, f64p, f64p, f64p,
                                          f64p]
         L.orc_estimate_sigma.restype = C.c_float
+        L.orc_set_spacing.argtypes = [C.c_float, C.c_float, C.c_float]
+        L.orc_w_edges.argtypes = [i32p, i32p, i32p, i32p, i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                  u32p, u32p]
         _LIB = L
     return _LIB
+
+
+def ref_available():
+    """Whether oracle/_ref was built, without loading it (so that collecting the tests
+    never maps the compiled reference into a process that does not call it)."""
+    return os.path.exists(os.path.join(_HERE, "_ref", "libroift_ref.so"))
 
 
 def ref_lib():
@@ -120,8 +129,29 @@ def ref_lib():
         L.ref_scene_read.argtypes = [C.c_char_p, i32p, C.c_longlong]
         L.ref_scene_write.argtypes = [C.c_char_p, i32p, C.c_int, C.c_int, C.c_int]
         L.ref_estimate_sigma.restype = C.c_float
+        L.ref_set_spacing.argtypes = [C.c_float, C.c_float, C.c_float]
+        L.ref_oift_w.argtypes = [i32p, i32p, i32p, i32p, i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                 i32p, i32p]
         _REF = L
     return _REF
+
+
+class _spacing:
+    """Sets the oracle's (and, with ref=True, the compiled reference's) voxel spacing for
+    the duration of a call; None leaves unit spacing."""
+
+    def __init__(self, spacing, ref=False):
+        self.sp = None if spacing is None else tuple(float(v) for v in spacing)
+        self.fn = (lambda: ref_lib().ref_set_spacing) if ref else (lambda: lib().orc_set_spacing)
+
+    def __enter__(self):
+        if self.sp is not None:
+            self.fn()(*self.sp)
+
+    def __exit__(self, *exc):
+        if self.sp is not None:
+            self.fn()(1.0, 1.0, 1.0)
+        return False
 
 
 def _dims(vol):
@@ -208,8 +238,9 @@ def oift_edges(wa, wb, seeds, label, adj_r=1.0, pol_sign=None, want_order=False)
     return out
 
 
-def relax(I, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None, fp64=False, dense=False):
-    """ORelax_1_Multi[_PerClass] restated.  Returns dict(label, memb[nc, ...], classes)."""
+def relax(I, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None, fp64=False, dense=False, spacing=None):
+    """ORelax_1_Multi[_PerClass] restated.  Returns dict(label, memb[nc, ...], classes).
+    spacing = (dx, dy, dz) reaches Dpq (None: unit)."""
     I = _i32(I)
     xs, ys, zs = _dims(I)
     lab = _i32(label).copy()
@@ -221,23 +252,25 @@ def relax(I, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None, fp64=False
     ncl = len(present)
     memb = np.empty((ncl,) + I.shape, np.float64 if fp64 else np.float32)
     fn = lib().orc_relax_f64 if fp64 else lib().orc_relax_f32
-    rc = fn(_p(I, C.c_int32), xs, ys, zs, adj_r, int(pc is not None), per, _p(pc, C.c_float),
-            -1 if pc is None else pc.size - 1, _p(S, C.c_int32), _p(lab, C.c_int32), int(niter), int(dense),
-            _p(memb, C.c_double if fp64 else C.c_float), _p(cls, C.c_int32), _p(nc, C.c_int32))
+    with _spacing(spacing):
+        rc = fn(_p(I, C.c_int32), xs, ys, zs, adj_r, int(pc is not None), per, _p(pc, C.c_float),
+                -1 if pc is None else pc.size - 1, _p(S, C.c_int32), _p(lab, C.c_int32), int(niter), int(dense),
+                _p(memb, C.c_double if fp64 else C.c_float), _p(cls, C.c_int32), _p(nc, C.c_int32))
     if rc != 0:
         raise ValueError(f"orc_relax failed rc={rc}")
     assert nc[0] == ncl
     return dict(label=lab.reshape(I.shape), memb=memb, classes=cls[: nc[0]].copy())
 
 
-def relax_binary(I, seeds, label, niter, adj_r=1.0, per=0.0):
+def relax_binary(I, seeds, label, niter, adj_r=1.0, per=0.0, spacing=None):
     I = _i32(I)
     xs, ys, zs = _dims(I)
     lab = _i32(label).copy()
     S = None if seeds is None else _gft_S(seeds)
     memb = np.empty(I.shape, np.float32)
-    rc = lib().orc_relax_binary_f32(_p(I, C.c_int32), xs, ys, zs, adj_r, per, _p(S, C.c_int32),
-                                    _p(lab, C.c_int32), int(niter), _p(memb, C.c_float))
+    with _spacing(spacing):
+        rc = lib().orc_relax_binary_f32(_p(I, C.c_int32), xs, ys, zs, adj_r, per, _p(S, C.c_int32),
+                                        _p(lab, C.c_int32), int(niter), _p(memb, C.c_float))
     assert rc == 0
     return dict(label=lab.reshape(I.shape), memb=memb)
 
@@ -246,7 +279,7 @@ RELAX_MODELS = {"intensity": 0, "grad": 1, "rbf": 2}
 
 
 def relax_model(I, seeds, label, niter, model, G=None, alpha=0.5, sigma=1.0, adj_r=1.0, per=0.0, fp64=False,
-                dense=False):
+                dense=False, spacing=None):
     """Multi-class relaxation with the GradWeight or Gaussian-RBF weight model restated
     (see orc_relax_model_f32).  Returns dict(label, memb[nc, ...], classes)."""
     I = _i32(I)
@@ -258,15 +291,16 @@ def relax_model(I, seeds, label, niter, model, G=None, alpha=0.5, sigma=1.0, adj
     present = {0} | {int(lab.ravel()[s]) for s in np.asarray(seeds).ravel() if lab.ravel()[s] >= 0}
     memb = np.empty((len(present),) + I.shape, np.float64 if fp64 else np.float32)
     fn = lib().orc_relax_model_f64 if fp64 else lib().orc_relax_model_f32
-    rc = fn(_p(I, C.c_int32), _p(Gi, C.c_int32), xs, ys, zs, adj_r, RELAX_MODELS[model], alpha, sigma, 0, per,
-            None, -1, _p(S, C.c_int32), _p(lab, C.c_int32), int(niter), int(dense),
-            _p(memb, C.c_double if fp64 else C.c_float), _p(cls, C.c_int32), _p(nc, C.c_int32))
+    with _spacing(spacing):
+        rc = fn(_p(I, C.c_int32), _p(Gi, C.c_int32), xs, ys, zs, adj_r, RELAX_MODELS[model], alpha, sigma, 0, per,
+                None, -1, _p(S, C.c_int32), _p(lab, C.c_int32), int(niter), int(dense),
+                _p(memb, C.c_double if fp64 else C.c_float), _p(cls, C.c_int32), _p(nc, C.c_int32))
     if rc != 0:
         raise ValueError(f"orc_relax_model failed rc={rc}")
     return dict(label=lab.reshape(I.shape), memb=memb[: nc[0]], classes=cls[: nc[0]].copy())
 
 
-def relax_binary_w(I, seeds, label, niter, adj_r=1.0, per=0.0, W=None, Wxyz=None, dense=False):
+def relax_binary_w(I, seeds, label, niter, adj_r=1.0, per=0.0, W=None, Wxyz=None, dense=False, spacing=None):
     """ORelax_1 in its image, per-voxel-weight (W) or per-axis (Wxyz) form."""
     I = _i32(I)
     xs, ys, zs = _dims(I)
@@ -275,9 +309,11 @@ def relax_binary_w(I, seeds, label, niter, adj_r=1.0, per=0.0, W=None, Wxyz=None
     Wi = None if W is None else _i32(W)
     Wx, Wy, Wz = (None, None, None) if Wxyz is None else tuple(_i32(w) for w in Wxyz)
     memb = np.empty(I.shape, np.float32)
-    rc = lib().orc_relax_binary_w_f32(_p(I, C.c_int32), _p(Wi, C.c_int32), _p(Wx, C.c_int32), _p(Wy, C.c_int32),
-                                      _p(Wz, C.c_int32), xs, ys, zs, adj_r, per, _p(S, C.c_int32),
-                                      _p(lab, C.c_int32), int(niter), int(dense), _p(memb, C.c_float))
+    with _spacing(spacing):
+        rc = lib().orc_relax_binary_w_f32(_p(I, C.c_int32), _p(Wi, C.c_int32), _p(Wx, C.c_int32),
+                                          _p(Wy, C.c_int32), _p(Wz, C.c_int32), xs, ys, zs, adj_r, per,
+                                          _p(S, C.c_int32), _p(lab, C.c_int32), int(niter), int(dense),
+                                          _p(memb, C.c_float))
     assert rc == 0
     return dict(label=lab.reshape(I.shape), memb=memb)
 
@@ -292,6 +328,28 @@ def gradweight_edges(I, G, adj_r=1.0, alpha=0.5, per=0.0):
                                     _p(wa, C.c_uint32), _p(wb, C.c_uint32))
     assert rc == 0
     return wa, wb
+
+
+def w_edges(I, W=None, Wxyz=None, adj_r=1.0, per=0.0):
+    """(wa, wb) of OIFT(W, ...) (base W_p + W_q) or OIFT(Wx, Wy, Wz, ...) (per-axis base,
+    Spheric(1.0)) in the orc_edge_weights layout (orc_w_edges)."""
+    I = _i32(I)
+    xs, ys, zs = _dims(I)
+    Wi = None if W is None else _i32(W)
+    Wx, Wy, Wz = (None, None, None) if Wxyz is None else tuple(_i32(w) for w in Wxyz)
+    E = (spheric(1.0 if Wxyz is not None else adj_r).shape[0] - 1) // 2
+    wa = np.empty((E,) + I.shape, np.uint32); wb = np.empty((E,) + I.shape, np.uint32)
+    rc = lib().orc_w_edges(_p(I, C.c_int32), _p(Wi, C.c_int32), _p(Wx, C.c_int32), _p(Wy, C.c_int32),
+                           _p(Wz, C.c_int32), xs, ys, zs, adj_r, per, _p(wa, C.c_uint32), _p(wb, C.c_uint32))
+    assert rc == 0
+    return wa, wb
+
+
+def oift_w(I, seeds, label, adj_r=1.0, per=0.0, W=None, Wxyz=None):
+    """OIFT(W, A, scn, ...) / OIFT(Wx, Wy, Wz, scn, ...) restated: W-form edge weights + the
+    keyed-heap OIFT (label 0 on the transposed weights, label > 0 on the stored ones)."""
+    wa, wb = w_edges(I, W, Wxyz, adj_r, per)
+    return oift_edges(wa, wb, seeds, label, 1.0 if Wxyz is not None else adj_r)
 
 
 def oift_gradweight(I, G, seeds, label, adj_r=1.0, alpha=0.5, per=0.0):
@@ -326,18 +384,20 @@ def relax_diff(I, seeds, label, lab_obj, niter, scale, sigma, alpha, target, adj
     return dict(loss=float(loss[0]), g_scale=gsc, g_sigma=float(gs[0]), g_alpha=float(ga[0]), f=f)
 
 
-def gauss_kernel():
+def gauss_kernel(spacing=None):
     v = np.zeros(7, np.float32)
-    n = lib().orc_gauss_kernel(_p(v, C.c_float))
+    with _spacing(spacing):
+        n = lib().orc_gauss_kernel(_p(v, C.c_float))
     return v[:n]
 
 
-def gaussian_blur(I, passes=1):
+def gaussian_blur(I, passes=1, spacing=None):
     """gft::Scene32::GaussianBlur applied `passes` times (orc_gaussian_blur)."""
     I = _i32(I)
     xs, ys, zs = _dims(I)
     out = np.empty_like(I)
-    rc = lib().orc_gaussian_blur(_p(I, C.c_int32), xs, ys, zs, int(passes), _p(out, C.c_int32))
+    with _spacing(spacing):
+        rc = lib().orc_gaussian_blur(_p(I, C.c_int32), xs, ys, zs, int(passes), _p(out, C.c_int32))
     assert rc == 0
     return out
 
@@ -404,14 +464,29 @@ def ref_oift(I, seeds, label, adj_r=1.0, per=0.0, per_class=None):
     return lab
 
 
-def ref_relax(I, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None):
+def ref_relax(I, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None, spacing=None):
     L = ref_lib()
     I = _i32(I); xs, ys, zs = _dims(I)
     lab = _i32(label).copy()
     S = _gft_S(seeds)
     pc = None if per_class is None else np.ascontiguousarray(per_class, np.float32)
-    L.ref_relax(_p(I, C.c_int32), xs, ys, zs, adj_r, int(pc is not None), per, _p(pc, C.c_float),
-                -1 if pc is None else pc.size - 1, _p(S, C.c_int32), _p(lab, C.c_int32), int(niter))
+    with _spacing(spacing, ref=True):
+        L.ref_relax(_p(I, C.c_int32), xs, ys, zs, adj_r, int(pc is not None), per, _p(pc, C.c_float),
+                    -1 if pc is None else pc.size - 1, _p(S, C.c_int32), _p(lab, C.c_int32), int(niter))
+    return lab
+
+
+def ref_oift_w(I, seeds, label, adj_r=1.0, per=0.0, W=None, Wxyz=None):
+    """The reference's gft::ift::OIFT: (A, scn, ...) with W = Wxyz = None, (W, A, scn, ...)
+    or (Wx, Wy, Wz, scn, ...)."""
+    L = ref_lib()
+    I = _i32(I); xs, ys, zs = _dims(I)
+    lab = _i32(label).copy()
+    S = None if seeds is None else _gft_S(seeds)
+    Wi = None if W is None else _i32(W)
+    Wx, Wy, Wz = (None, None, None) if Wxyz is None else tuple(_i32(w) for w in Wxyz)
+    L.ref_oift_w(_p(I, C.c_int32), _p(Wi, C.c_int32), _p(Wx, C.c_int32), _p(Wy, C.c_int32), _p(Wz, C.c_int32),
+                 xs, ys, zs, adj_r, per, _p(S, C.c_int32), _p(lab, C.c_int32))
     return lab
 
 
@@ -427,15 +502,17 @@ def ref_pipeline(I, seeds, label, niter, adj_r=1.0, per=0.0):
     return lab, float(t1[0]), float(t2[0])
 
 
-def ref_relax_binary(I, seeds, label, niter, adj_r=1.0, per=0.0, W=None, Wxyz=None):
+def ref_relax_binary(I, seeds, label, niter, adj_r=1.0, per=0.0, W=None, Wxyz=None, spacing=None):
     L = ref_lib()
     I = _i32(I); xs, ys, zs = _dims(I)
     lab = _i32(label).copy()
     S = _gft_S(np.zeros(0, np.int32) if seeds is None else seeds)
     Wi = None if W is None else _i32(W)
     Wx, Wy, Wz = (None, None, None) if Wxyz is None else tuple(_i32(w) for w in Wxyz)
-    L.ref_relax_binary(_p(I, C.c_int32), _p(Wi, C.c_int32), _p(Wx, C.c_int32), _p(Wy, C.c_int32),
-                       _p(Wz, C.c_int32), xs, ys, zs, adj_r, per, _p(S, C.c_int32), _p(lab, C.c_int32), int(niter))
+    with _spacing(spacing, ref=True):
+        L.ref_relax_binary(_p(I, C.c_int32), _p(Wi, C.c_int32), _p(Wx, C.c_int32), _p(Wy, C.c_int32),
+                           _p(Wz, C.c_int32), xs, ys, zs, adj_r, per, _p(S, C.c_int32), _p(lab, C.c_int32),
+                           int(niter))
     return lab
 
 
@@ -448,21 +525,23 @@ def ref_oift_gradweight(I, G, seeds, label, adj_r=1.0, alpha=0.5, per=0.0):
     return lab
 
 
-def ref_relax_gradweight(I, G, seeds, label, niter, adj_r=1.0, alpha=0.5, per=0.0):
+def ref_relax_gradweight(I, G, seeds, label, niter, adj_r=1.0, alpha=0.5, per=0.0, spacing=None):
     L = ref_lib()
     I = _i32(I); Gi = _i32(G); xs, ys, zs = _dims(I)
     lab = _i32(label).copy()
-    L.ref_relax_gradweight(_p(I, C.c_int32), _p(Gi, C.c_int32), xs, ys, zs, adj_r, per, alpha,
-                           _p(_gft_S(seeds), C.c_int32), _p(lab, C.c_int32), int(niter))
+    with _spacing(spacing, ref=True):
+        L.ref_relax_gradweight(_p(I, C.c_int32), _p(Gi, C.c_int32), xs, ys, zs, adj_r, per, alpha,
+                               _p(_gft_S(seeds), C.c_int32), _p(lab, C.c_int32), int(niter))
     return lab
 
 
-def ref_relax_rbf(I, seeds, label, niter, sigma, adj_r=1.0, per=0.0):
+def ref_relax_rbf(I, seeds, label, niter, sigma, adj_r=1.0, per=0.0, spacing=None):
     L = ref_lib()
     I = _i32(I); xs, ys, zs = _dims(I)
     lab = _i32(label).copy()
-    L.ref_relax_rbf(_p(I, C.c_int32), xs, ys, zs, adj_r, per, sigma, _p(_gft_S(seeds), C.c_int32),
-                    _p(lab, C.c_int32), int(niter))
+    with _spacing(spacing, ref=True):
+        L.ref_relax_rbf(_p(I, C.c_int32), xs, ys, zs, adj_r, per, sigma, _p(_gft_S(seeds), C.c_int32),
+                        _p(lab, C.c_int32), int(niter))
     return lab
 
 
@@ -471,16 +550,18 @@ def ref_estimate_sigma(I):
     return float(ref_lib().ref_estimate_sigma(_p(I, C.c_int32), xs, ys, zs))
 
 
-def ref_gaussian_blur(I, passes=1):
+def ref_gaussian_blur(I, passes=1, spacing=None):
     I = _i32(I); xs, ys, zs = _dims(I)
     out = np.empty_like(I)
-    ref_lib().ref_gaussian_blur(_p(I, C.c_int32), xs, ys, zs, int(passes), _p(out, C.c_int32))
+    with _spacing(spacing, ref=True):
+        ref_lib().ref_gaussian_blur(_p(I, C.c_int32), xs, ys, zs, int(passes), _p(out, C.c_int32))
     return out
 
 
-def ref_gauss_kernel():
+def ref_gauss_kernel(spacing=None):
     v = np.zeros(8, np.float32)
-    n = ref_lib().ref_gauss_kernel(_p(v, C.c_float), 8)
+    with _spacing(spacing, ref=True):
+        n = ref_lib().ref_gauss_kernel(_p(v, C.c_float), 8)
     return v[:n]
 
 

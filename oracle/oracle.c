@@ -30,6 +30,14 @@
  *                           OIFT_Multi_GradWeight labels; golden fixtures.
  *   orc_relax_binary_w   -- oracle/_ref labels (ORelax_1 image / W / Wxyz
  *                           forms); golden fixtures; constant-image closed form.
+ *   orc_w_edges          -- orc_oift_edges on them == the compiled OIFT(W, ...)
+ *                           and OIFT(Wx, Wy, Wz, ...) labels (golden fixtures);
+ *                           W = |I| differences reduce it to orc_edge_weights
+ *                           with the base doubled; exhaustive factor identity.
+ *   orc_set_spacing      -- the relaxations' Dpq and the blur taps under
+ *                           anisotropic spacing: golden fixtures written by the
+ *                           compiled reference; isotropic spacing of any size
+ *                           equals unit spacing bit for bit; the taps' closed form.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -43,6 +51,24 @@
 #define ORC_BLACK 2
 
 enum { ORC_OK = 0, ORC_EINVAL = -1, ORC_ENOMEM = -2 };
+
+/* Voxel spacing (gft's scn->dx/dy/dz) read by the relaxations' Dpq and the Gaussian blur's
+   taps; Scene32::Create's default is 1 (gft_scene32.cpp:27-29).  Set by orc_set_spacing
+   before a call (the Python wrapper restores 1, 1, 1 after it). */
+static float g_sp[3] = {1.0f, 1.0f, 1.0f};
+
+int orc_set_spacing(float dx, float dy, float dz)
+{
+    g_sp[0] = dx; g_sp[1] = dy; g_sp[2] = dz;
+    return ORC_OK;
+}
+
+static float sp_min(void)
+{
+    /* MIN(scn->dx, MIN(scn->dy, scn->dz)) */
+    float m = g_sp[1] < g_sp[2] ? g_sp[1] : g_sp[2];
+    return g_sp[0] < m ? g_sp[0] : m;
+}
 
 /* ------------------------------------------------------------------------ */
 /* Adjacency: gft::AdjRel3::Spheric(r) (gft_adjrel3.cpp:47-85).              */
@@ -305,6 +331,52 @@ int orc_edge_weights(const int32_t *I, int xs, int ys, int zs, float adj_r, floa
 }
 
 /* ------------------------------------------------------------------------ */
+/* Oriented edge weights of the per-voxel weight forms OIFT(W, A, scn, ...)  */
+/* (gft_ift.cpp:1142-1152) and OIFT(Wx, Wy, Wz, scn, ...) (:1244-1254): the   */
+/* int base W[p] + W[q] (per-axis: the W of the arc's axis), times           */
+/* (1 +- per_pq/100.0) in double by the order of I[p] and I[q], truncated;    */
+/* unchanged when I[p] == I[q].  The base is symmetric, so the orc_edge_     */
+/* weights layout (wa = w_obj(p->q), wb = w_obj(q->p)) holds.  The per-axis   */
+/* form is defined on Spheric(1.0).                                          */
+/* ------------------------------------------------------------------------ */
+static inline int w_arc_weight(int base, int Ip, int Iq, float per_pq)
+{
+    int w = base;
+    if (Ip > Iq) w = (int)((double)w * (1.0 + per_pq / 100.0));
+    else if (Ip < Iq) w = (int)((double)w * (1.0 - per_pq / 100.0));
+    return w;
+}
+
+int orc_w_edges(const int32_t *I, const int32_t *W, const int32_t *Wx, const int32_t *Wy, const int32_t *Wz,
+                int xs, int ys, int zs, float adj_r, float per, uint32_t *wa, uint32_t *wb)
+{
+    orc_adj A;
+    if (Wx && (!Wy || !Wz)) return ORC_EINVAL;
+    if (!W && !Wx) return ORC_EINVAL;
+    if (adj_make(Wx ? 1.0f : adj_r, &A) != ORC_OK) return ORC_EINVAL;
+    const int half = (A.n - 1) / 2;
+    const int64_t n = (int64_t)xs * ys * zs;
+    const int xy = xs * ys;
+    for (int s = 0; s < half; s++) {
+        int i = half + 1 + s;
+        const int32_t *B = W;
+        if (Wx) B = A.dz[i] != 0 ? Wz : (A.dy[i] != 0 ? Wy : Wx);
+        for (int64_t p = 0; p < n; p++) {
+            int px = (int)((p % xy) % xs), py = (int)((p % xy) / xs), pz = (int)(p / xy);
+            int qx = px + A.dx[i], qy = py + A.dy[i], qz = pz + A.dz[i];
+            if (qx < 0 || qx >= xs || qy < 0 || qy >= ys || qz < 0 || qz >= zs) {
+                wa[s * n + p] = 0xFFFFFFFFu; wb[s * n + p] = 0xFFFFFFFFu; continue;
+            }
+            int64_t q = qx + (int64_t)qy * xs + (int64_t)qz * xy;
+            const int base = B[p] + B[q];
+            wa[s * n + p] = (uint32_t)w_arc_weight(base, I[p], I[q], per);
+            wb[s * n + p] = (uint32_t)w_arc_weight(base, I[q], I[p], per);
+        }
+    }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Oriented edge weights of OIFT_Multi_GradWeight (exp_gradient_weight.cpp:  */
 /* 244-254): base wf = alpha|I_p-I_q| + (1-alpha)(G_p+G_q)*0.5 in float,      */
 /* w = (int)(wf (1 +- per_pq/100)) in float, (int)wf when I_p == I_q, and a   */
@@ -528,9 +600,10 @@ done:
     }                                                                                         \
     REAL Dpq[125];                                                                            \
     for (int i = 1; i < A.n; i++) {                                                           \
-        /* spacing dx=dy=dz=1 (Scene32::Create defaults, gft_scene32.cpp:28-30) */            \
+        /* sqrtf(x^2 dx^2 + y^2 dy^2 + z^2 dz^2) / dmin (gft_ift.cpp:8723-8731) */            \
         REAL ax = (REAL)(A.dx[i] * A.dx[i]), ay = (REAL)(A.dy[i] * A.dy[i]), az = (REAL)(A.dz[i] * A.dz[i]); \
-        Dpq[i] = SQRT(ax * (REAL)1 * (REAL)1 + ay * (REAL)1 * (REAL)1 + az * (REAL)1 * (REAL)1) / (REAL)1; \
+        Dpq[i] = SQRT(ax * (REAL)g_sp[0] * (REAL)g_sp[0] + ay * (REAL)g_sp[1] * (REAL)g_sp[1] +  \
+                      az * (REAL)g_sp[2] * (REAL)g_sp[2]) / (REAL)sp_min();                    \
     }                                                                                         \
     int Imax = INT_MIN;                                                                       \
     for (int p = 0; p < n; p++) if (I[p] > Imax) Imax = I[p];                                  \
@@ -715,9 +788,11 @@ int orc_relax_binary_w_f32(const int32_t *I, const int32_t *W, const int32_t *Wx
     if (n64 > INT_MAX || n64 <= 0) return ORC_EINVAL;
     if (Wx && (!Wy || !Wz)) return ORC_EINVAL;
     const int n = (int)n64, xy = xs * ys;
-    float Dpq[125];
+    float Dpq[125];   /* gft_ift.cpp:8515-8522 */
     for (int i = 1; i < A.n; i++)
-        Dpq[i] = sqrtf((float)(A.dx[i] * A.dx[i]) + (float)(A.dy[i] * A.dy[i]) + (float)(A.dz[i] * A.dz[i]));
+        Dpq[i] = sqrtf((float)(A.dx[i] * A.dx[i]) * g_sp[0] * g_sp[0] + (float)(A.dy[i] * A.dy[i]) * g_sp[1] * g_sp[1] +
+                       (float)(A.dz[i] * A.dz[i]) * g_sp[2] * g_sp[2]) /
+                 sp_min();
     int axis[125];   /* 0 = x, 1 = y, 2 = z (the last non-zero component wins, as gft_ift.cpp:9356-9364) */
     for (int i = 1; i < A.n; i++) {
         axis[i] = 0;
@@ -931,8 +1006,9 @@ int orc_relax_diff_f64(const int32_t *I, int xs, int ys, int zs, float adj_r, co
 /* gft::Scene32::GaussianBlur (gft_filtering3.cpp:385-397): convolution     */
 /* with Kernel3::SphericalGaussian(scn, R=1, s=10, f=1) normalised          */
 /* (gft_filtering3.cpp:72-126): taps in Spheric(1.0) order, centre first,   */
-/* val = (float)(s * exp((double)(-f * (r2 / R2)))) (exp of a double: the   */
-/* float argument is promoted), normalised by their float sum.  The         */
+/* val = s * expf(-f * (r2 / R2)) in float (C++'s exp(float) overload),     */
+/* with r2 from the spacing-scaled displacements, normalised by their float */
+/* sum.  The                                                                */
 /* convolution sums (float)I[q] * val in tap order over the taps inside the */
 /* volume (no renormalisation at the border) and rounds with gft's ROUND    */
 /* (gft_common.h:120, the 0.5 is a double) (gft_filtering3.cpp:321-346).    */
@@ -941,11 +1017,17 @@ int orc_gauss_kernel(float *val)
 {
     orc_adj A;
     if (adj_make(1.0f, &A) != ORC_OK) return ORC_EINVAL;
-    const float R = 1.0f, s = 10.0f, f = 1.0f, R2 = R * R;
+    const float R = 1.0f, s = 10.0f, f = 1.0f, R2 = R * R, dmin = sp_min();
     float wt = 0.0f;
     for (int i = 0; i < A.n; i++) {
-        const float r2 = (float)(A.dx[i] * A.dx[i] + A.dy[i] * A.dy[i] + A.dz[i] * A.dz[i]);
-        val[i] = (float)((double)s * exp((double)(-f * (r2 / R2))));
+        /* displacements scaled by spacing / dmin in float (gft_filtering3.cpp:114-120) */
+        const float d_x = ((float)A.dx[i] * g_sp[0]) / dmin;
+        const float d_y = ((float)A.dy[i] * g_sp[1]) / dmin;
+        const float d_z = ((float)A.dz[i] * g_sp[2]) / dmin;
+        const float r2 = d_x * d_x + d_y * d_y + d_z * d_z;
+        /* exp of a float: C++ resolves exp(float) to the float overload; at unit spacing
+           (r2 in {0, 1}) the double form rounds to the same taps */
+        val[i] = s * expf(-f * (r2 / R2));
     }
     for (int i = 0; i < A.n; i++) wt += val[i];
     for (int i = 0; i < A.n; i++) val[i] /= wt;

@@ -12,10 +12,16 @@
 
 namespace {
 
+// voxel spacing given to every scene made here (ref_set_spacing; gft's default is 1, 1, 1)
+float g_spacing[3] = {1.0f, 1.0f, 1.0f};
+
 gft::sScene32 *make_scene(const int *data, int xs, int ys, int zs)
 {
     gft::sScene32 *s = gft::Scene32::Create(xs, ys, zs);
     std::memcpy(s->data, data, sizeof(int) * (size_t)s->n);
+    s->dx = g_spacing[0];
+    s->dy = g_spacing[1];
+    s->dz = g_spacing[2];
     return s;
 }
 
@@ -50,6 +56,52 @@ REF_API int ref_oift(const int *I, int xs, int ys, int zs, float adj_r,
         gft::ift::OIFT_Multi_PerClass(A, scn, per_class, max_label, Sc, lab);
     else
         gft::ift::OIFT_Multi(A, scn, per, Sc, lab);
+    std::memcpy(label, lab->data, sizeof(int) * (size_t)lab->n);
+    std::free(Sc);
+    gft::Scene32::Destroy(&lab);
+    gft::Scene32::Destroy(&scn);
+    gft::AdjRel3::Destroy(&A);
+    return 0;
+}
+
+void ref_get_spacing(float *out3)
+{
+    out3[0] = g_spacing[0];
+    out3[1] = g_spacing[1];
+    out3[2] = g_spacing[2];
+}
+
+REF_API int ref_set_spacing(float dx, float dy, float dz)
+{
+    g_spacing[0] = dx;
+    g_spacing[1] = dy;
+    g_spacing[2] = dz;
+    return 0;
+}
+
+// The two-label outer cut gft::ift::OIFT(A, scn, per, S, label) (gft_ift.cpp:828-908) and the
+// per-voxel weight forms OIFT(W, A, scn, ...) (:1086-1167) / OIFT(Wx, Wy, Wz, scn, ...)
+// (:1170-1271): W == Wx == NULL runs the first.  label is in/out.
+REF_API int ref_oift_w(const int *I, const int *W, const int *Wx, const int *Wy, const int *Wz, int xs, int ys,
+                       int zs, float adj_r, float per, const int *S, int *label)
+{
+    gft::sAdjRel3 *A = gft::AdjRel3::Spheric(adj_r);
+    gft::sScene32 *scn = make_scene(I, xs, ys, zs);
+    gft::sScene32 *lab = make_scene(label, xs, ys, zs);
+    int *Sc = S ? make_S(S) : nullptr;
+    if (Wx) {
+        gft::sScene32 *sx = make_scene(Wx, xs, ys, zs), *sy = make_scene(Wy, xs, ys, zs), *sz = make_scene(Wz, xs, ys, zs);
+        gft::ift::OIFT(sx, sy, sz, scn, per, Sc, lab);
+        gft::Scene32::Destroy(&sx);
+        gft::Scene32::Destroy(&sy);
+        gft::Scene32::Destroy(&sz);
+    } else if (W) {
+        gft::sScene32 *sw = make_scene(W, xs, ys, zs);
+        gft::ift::OIFT(sw, A, scn, per, Sc, lab);
+        gft::Scene32::Destroy(&sw);
+    } else {
+        gft::ift::OIFT(A, scn, per, Sc, lab);
+    }
     std::memcpy(label, lab->data, sizeof(int) * (size_t)lab->n);
     std::free(Sc);
     gft::Scene32::Destroy(&lab);
@@ -128,6 +180,9 @@ REF_API int ref_gaussian_blur(const int *I, int xs, int ys, int zs, int passes, 
 REF_API int ref_gauss_kernel(float *vals, int cap)
 {
     gft::sScene32 *scn = gft::Scene32::Create(3, 3, 3);
+    scn->dx = g_spacing[0];
+    scn->dy = g_spacing[1];
+    scn->dz = g_spacing[2];
     gft::sKernel3 *K = gft::Kernel3::SphericalGaussian(scn, 1.0f, 10.0f, 1.0f);
     gft::sKernel3 *NK = gft::Kernel3::Normalize(K);
     const int n = NK->adj->n;

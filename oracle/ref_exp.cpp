@@ -41,12 +41,20 @@ namespace exp_rbf {
 #include EXP_RBF
 }
 
+// the voxel spacing ref_set_spacing gave (ref_capi.cpp)
+extern "C" void ref_get_spacing(float *out3);
+
 namespace {
 
 gft::sScene32 *scene_of(const int *data, int xs, int ys, int zs)
 {
     gft::sScene32 *s = gft::Scene32::Create(xs, ys, zs);
     std::memcpy(s->data, data, sizeof(int) * (size_t)s->n);
+    float sp[3];
+    ref_get_spacing(sp);
+    s->dx = sp[0];
+    s->dy = sp[1];
+    s->dz = sp[2];
     return s;
 }
 

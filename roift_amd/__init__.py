@@ -26,13 +26,14 @@ EXPORTS = (
     "roift_create", "roift_destroy", "roift_last_error", "roift_set_stream", "roift_last_timings",
     "roift_kernel_timings", "roift_oift_profile", "roift_set_param",
     "roift_weights_dev", "roift_oift_fmax_dev", "roift_oift_edges_dev", "roift_relax_dev",
-    "roift_pipeline_dev", "roift_oift_fmax", "roift_relax", "oift_fmax_gpu_run",
+    "roift_pipeline_dev", "roift_oift_fmax", "roift_relax", "oift_fmax_gpu_run", "oift_gpu_run",
     "roift_relax_binary_dev", "roift_oift_gradweight_dev", "roift_relax_gradweight_dev", "roift_relax_rbf_dev",
     "roift_relax_binary", "roift_batch_pipeline_dev", "roift_relax_adjoint_dev",
     "roift_gaussian_blur_dev", "roift_gaussian_blur",
     "roift_sobel_dev", "roift_bfs_edt_dev", "roift_boundary_seeds_dev", "roift_dilation_conditional_dev",
     "roift_io_last_error", "roift_volume_info_read", "roift_volume_read", "roift_volume_write", "roift_seeds_read",
     "roift_polarity_read", "roift_volume_read_dev", "roift_cli_main", "roift_relax_adjoint64_dev",
+    "roift_weights_w_dev", "roift_oift_w_dev", "roift_oift_w",
 )
 
 
@@ -55,7 +56,7 @@ READ_SHIFT_MIN = 1
 class Problem(C.Structure):
     _fields_ = [("xsize", C.c_int), ("ysize", C.c_int), ("zsize", C.c_int), ("adj_radius", C.c_float),
                 ("use_per_class", C.c_int), ("per", C.c_float), ("h_per_class", C.POINTER(C.c_float)),
-                ("max_label", C.c_int)]
+                ("max_label", C.c_int), ("dx", C.c_float), ("dy", C.c_float), ("dz", C.c_float)]
 
 
 _lib = None
@@ -90,11 +91,15 @@ def lib():
         L.roift_weights_dev.argtypes = [vp, pp, vp, ip, vp, vp, ip, i32p, i32p]
         L.roift_oift_fmax_dev.argtypes = [vp, pp, vp, ip, vp, ip, vp, vp]
         L.roift_oift_edges_dev.argtypes = [vp, pp, vp, vp, ip, i8p, vp, ip, vp, vp]
+        L.roift_weights_w_dev.argtypes = [vp, pp, vp, ip, vp, vp, vp, vp, vp, vp, ip]
+        L.roift_oift_w_dev.argtypes = [vp, pp, vp, ip, vp, vp, vp, vp, vp, ip, vp, vp]
+        L.roift_oift_w.argtypes = [vp, pp, vp, vp, vp, vp, vp, vp, ip, vp, vp]
         L.roift_relax_dev.argtypes = [vp, pp, vp, ip, vp, ip, vp, ip, vp, i32p]
         L.roift_pipeline_dev.argtypes = [vp, pp, vp, ip, vp, ip, vp, ip, vp, vp]
         L.roift_oift_fmax.argtypes = [vp, pp, vp, vp, ip, vp, vp]
         L.roift_relax.argtypes = [vp, pp, vp, vp, ip, vp, ip, vp]
         L.oift_fmax_gpu_run.argtypes = [vp, vp, ip, ip, ip, ip, vp, ip, vp, ip]
+        L.oift_gpu_run.argtypes = [ip, vp, vp, ip, ip, ip, ip, vp, ip, vp, ip]
         fl, fp = C.c_float, C.POINTER(C.c_float)
         L.roift_relax_binary_dev.argtypes = [vp, pp, vp, ip, vp, vp, vp, vp, vp, ip, vp, ip, vp]
         L.roift_oift_gradweight_dev.argtypes = [vp, pp, vp, ip, vp, fl, vp, ip, vp, vp]
@@ -188,10 +193,11 @@ def cli_main(args):
     return lib().roift_cli_main(len(argv), arr)
 
 
-def make_problem(shape, adj_r=1.0, per=0.0, per_class=None):
-    """shape = (zs, ys, xs) of a C-order volume."""
+def make_problem(shape, adj_r=1.0, per=0.0, per_class=None, spacing=None):
+    """shape = (zs, ys, xs) of a C-order volume; spacing = (dx, dy, dz) (None: unit spacing)."""
     zs, ys, xs = (int(v) for v in shape)
-    pb = Problem(xs, ys, zs, float(adj_r), 0, float(per), None, -1)
+    dx, dy, dz = (1.0, 1.0, 1.0) if spacing is None else (float(v) for v in spacing)
+    pb = Problem(xs, ys, zs, float(adj_r), 0, float(per), None, -1, dx, dy, dz)
     keep = None
     if per_class is not None:
         keep = np.ascontiguousarray(per_class, np.float32)
@@ -220,11 +226,27 @@ class Context:
         self.device = device
 
     def close(self):
+        """Release the context's device memory and stream (roift_destroy).  Call it, or use
+        the context as a `with` block, before the program ends."""
         if getattr(self, "_h", None):
             self._L.roift_destroy(self._h)
             self._h = None
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
     def __del__(self):
+        # During interpreter shutdown the HIP runtime (and a profiler's library wrapped
+        # around it) may already be torn down: a HIP call from here would run after
+        # them.  The process exit releases the device memory anyway.
+        import sys
+        if sys.is_finalizing():
+            self._h = None
+            return
         try:
             self.close()
         except Exception:
@@ -296,6 +318,22 @@ class Context:
                                                 cost.data_ptr() if cost is not None else None))
         return cost
 
+    def oift_w(self, image, seeds, label, adj_r=1.0, per=0.0, W=None, Wxyz=None, want_cost=True):
+        """gft::ift::OIFT(W, A, scn, ...) (W: int32 device tensor) or OIFT(Wx, Wy, Wz, scn, ...)
+        (Wxyz: three int32 device tensors, Spheric(1.0)) in place on `label`; returns the cost map."""
+        import torch
+        assert label.dtype == torch.int32 and label.is_contiguous() and image.is_contiguous()
+        pb, _ = make_problem(image.shape, 1.0 if Wxyz is not None else adj_r, per)
+        seeds = seeds.to(device=label.device, dtype=torch.int32).contiguous()
+        cost = torch.empty_like(label) if want_cost else None
+        ws = [None if W is None else W.to(torch.int32).contiguous()]
+        ws += [None] * 3 if Wxyz is None else [w.to(torch.int32).contiguous() for w in Wxyz]
+        ptr = [None if w is None else w.data_ptr() for w in ws]
+        self._check(self._L.roift_oift_w_dev(self._h, C.byref(pb), image.data_ptr(), _dtype_code(image), *ptr,
+                                             seeds.data_ptr(), int(seeds.numel()), label.data_ptr(),
+                                             cost.data_ptr() if cost is not None else None))
+        return cost
+
     def oift_edges(self, wa, wb, seeds, label, adj_r=1.0, pol_sign=None, want_cost=True):
         import torch
         pb, _ = make_problem(label.shape, adj_r)
@@ -312,10 +350,10 @@ class Context:
             label.data_ptr(), cost.data_ptr() if cost is not None else None))
         return cost
 
-    def relax(self, image, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None, want_memb=True):
+    def relax(self, image, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None, want_memb=True, spacing=None):
         """In place on `label`; returns memberships [nclasses, zs, ys, xs] (float32) or None."""
         import torch
-        pb, _keep = make_problem(image.shape, adj_r, per, per_class)
+        pb, _keep = make_problem(image.shape, adj_r, per, per_class, spacing)
         seeds = seeds.to(device=label.device, dtype=torch.int32).contiguous()
         memb = torch.empty((8,) + tuple(label.shape), dtype=torch.float32, device=label.device) if want_memb else None
         nc = C.c_int32(0)
@@ -324,10 +362,11 @@ class Context:
                                             memb.data_ptr() if memb is not None else None, C.byref(nc)))
         return memb[: nc.value] if memb is not None else None
 
-    def relax_binary(self, image, seeds, label, niter, adj_r=1.0, per=0.0, W=None, Wxyz=None, want_memb=True):
+    def relax_binary(self, image, seeds, label, niter, adj_r=1.0, per=0.0, W=None, Wxyz=None, want_memb=True,
+                     spacing=None):
         """ORelax_1 (image, W or Wx/Wy/Wz form), in place on `label` (0/1 out); returns f or None."""
         import torch
-        pb, _ = make_problem(image.shape, adj_r, per)
+        pb, _ = make_problem(image.shape, adj_r, per, spacing=spacing)
         seeds = (torch.zeros(0, dtype=torch.int32, device=label.device) if seeds is None
                  else seeds.to(device=label.device, dtype=torch.int32).contiguous())
         memb = torch.empty(tuple(label.shape), dtype=torch.float32, device=label.device) if want_memb else None
@@ -343,16 +382,17 @@ class Context:
                                                    memb.data_ptr() if memb is not None else None))
         return memb
 
-    def gaussian_blur(self, image, passes=2, out=None):
-        """gft::Scene32::GaussianBlur applied `passes` times to an int16/int32 device volume.
-        Returns `out` (a new tensor of the input's dtype when None; may be `image` itself)."""
+    def gaussian_blur(self, image, passes=2, out=None, spacing=None):
+        """gft::Scene32::GaussianBlur applied `passes` times to an int16/int32 device volume
+        of voxel spacing (dx, dy, dz) (None: unit).  Returns `out` (a new tensor of the input's
+        dtype when None; may be `image` itself)."""
         import torch
         image = image.contiguous()
         if out is None:
             out = torch.empty_like(image)
         if out.dtype != image.dtype or out.shape != image.shape or not out.is_contiguous():
             raise ValueError("out must be a contiguous tensor of the image's dtype and shape")
-        pb, _ = make_problem(image.shape)
+        pb, _ = make_problem(image.shape, spacing=spacing)
         self._check(self._L.roift_gaussian_blur_dev(self._h, C.byref(pb), image.data_ptr(), _dtype_code(image),
                                                     out.data_ptr(), int(passes)))
         return out
@@ -370,10 +410,10 @@ class Context:
                                                   C.byref(info)))
         return out, info, shift.value
 
-    def gaussian_blur_host(self, image, passes=2):
+    def gaussian_blur_host(self, image, passes=2, spacing=None):
         img = np.ascontiguousarray(image, np.int32)
         out = np.empty_like(img)
-        pb, _ = make_problem(img.shape)
+        pb, _ = make_problem(img.shape, spacing=spacing)
         self._check(self._L.roift_gaussian_blur(self._h, C.byref(pb), img.ctypes.data, out.ctypes.data, int(passes)))
         return out
 
@@ -438,10 +478,11 @@ class Context:
                                                       cost.data_ptr() if cost is not None else None))
         return cost
 
-    def relax_gradweight(self, image, grad, seeds, label, niter, alpha=0.5, adj_r=1.0, per=0.0, want_memb=True):
+    def relax_gradweight(self, image, grad, seeds, label, niter, alpha=0.5, adj_r=1.0, per=0.0, want_memb=True,
+                         spacing=None):
         """ORelax_1_Multi_GradWeight in place on `label`; returns memberships [nclasses, ...] or None."""
         import torch
-        pb, _ = make_problem(image.shape, adj_r, per)
+        pb, _ = make_problem(image.shape, adj_r, per, spacing=spacing)
         seeds = seeds.to(device=label.device, dtype=torch.int32).contiguous()
         g = grad.to(torch.int32).contiguous()
         nc = _nclasses(label, seeds)
@@ -452,11 +493,11 @@ class Context:
                                                        memb.data_ptr() if memb is not None else None))
         return memb
 
-    def relax_rbf(self, image, seeds, label, niter, sigma=0.0, adj_r=1.0, want_memb=True):
+    def relax_rbf(self, image, seeds, label, niter, sigma=0.0, adj_r=1.0, want_memb=True, spacing=None):
         """ORelax_1_Multi_GaussRBF in place on `label`; sigma <= 0 uses the image's global std.
         Returns (memberships or None, sigma used)."""
         import torch
-        pb, _ = make_problem(image.shape, adj_r, 0.0)
+        pb, _ = make_problem(image.shape, adj_r, 0.0, spacing=spacing)
         seeds = seeds.to(device=label.device, dtype=torch.int32).contiguous()
         nc = _nclasses(label, seeds)
         memb = torch.empty((nc,) + tuple(label.shape), dtype=torch.float32, device=label.device) if want_memb else None
@@ -514,9 +555,10 @@ class Context:
             memb.data_ptr() if memb is not None else None))
         return loss.value, gsc, gs.value, ga.value, memb
 
-    def pipeline(self, image, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None, cost=None, memb=None):
+    def pipeline(self, image, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None, cost=None, memb=None,
+                 spacing=None):
         """weights + OIFT + relax + labels on device tensors (label in/out)."""
-        pb, _keep = make_problem(image.shape, adj_r, per, per_class)
+        pb, _keep = make_problem(image.shape, adj_r, per, per_class, spacing)
         self._check(self._L.roift_pipeline_dev(self._h, C.byref(pb), image.data_ptr(), _dtype_code(image),
                                                seeds.data_ptr(), int(seeds.numel()), label.data_ptr(), int(niter),
                                                cost.data_ptr() if cost is not None else None,
@@ -533,25 +575,25 @@ class Context:
                                             lab.ctypes.data, None if cost is None else cost.ctypes.data))
         return lab, cost
 
-    def relax_binary_host(self, image, seeds, label, niter, adj_r=1.0, per=0.0, W=None, Wxyz=None):
+    def relax_binary_host(self, image, seeds, label, niter, adj_r=1.0, per=0.0, W=None, Wxyz=None, spacing=None):
         img = np.ascontiguousarray(image, np.int32)
         lab = np.ascontiguousarray(label, np.int32).copy()
         S = np.ascontiguousarray(np.zeros(0) if seeds is None else seeds, np.int32)
         ws = [None if W is None else np.ascontiguousarray(W, np.int32)]
         ws += [None] * 3 if Wxyz is None else [np.ascontiguousarray(w, np.int32) for w in Wxyz]
         memb = np.empty(img.shape, np.float32)
-        pb, _ = make_problem(img.shape, adj_r, per)
+        pb, _ = make_problem(img.shape, adj_r, per, spacing=spacing)
         self._check(self._L.roift_relax_binary(self._h, C.byref(pb), img.ctypes.data,
                                                *[None if w is None else w.ctypes.data for w in ws],
                                                S.ctypes.data, S.size, lab.ctypes.data, int(niter), memb.ctypes.data))
         return lab, memb
 
-    def relax_host(self, image, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None):
+    def relax_host(self, image, seeds, label, niter, adj_r=1.0, per=0.0, per_class=None, spacing=None):
         img = np.ascontiguousarray(image, np.int32)
         lab = np.ascontiguousarray(label, np.int32).copy()
         S = np.ascontiguousarray(seeds, np.int32)
         memb = np.empty((8,) + img.shape, np.float32)
-        pb, _keep = make_problem(img.shape, adj_r, per, per_class)
+        pb, _keep = make_problem(img.shape, adj_r, per, per_class, spacing)
         self._check(self._L.roift_relax(self._h, C.byref(pb), img.ctypes.data, S.ctypes.data, S.size,
                                         lab.ctypes.data, int(niter), memb.ctypes.data))
         return lab, memb
@@ -575,3 +617,14 @@ def oift_fmax_gpu_run(h_image, h_labels, xsize, ysize, zsize, per_class, seeds):
     S = np.ascontiguousarray(seeds, np.int32)
     return lib().oift_fmax_gpu_run(img.ctypes.data, h_labels.ctypes.data, img.size, xsize, ysize, zsize,
                                    pc.ctypes.data, pc.size - 1, S.ctypes.data, S.size)
+
+
+def oift_gpu_run(cost_mode, h_image, h_labels, xsize, ysize, zsize, per_class, seeds):
+    """The reference's second legacy C entry point (gpu/oift_gpu.cu:149-158), h_labels in/out;
+    cost_mode 0 (f_max) only, any other mode returns -1."""
+    img = np.ascontiguousarray(h_image, np.int32)
+    assert h_labels.dtype == np.int32 and h_labels.flags.c_contiguous
+    pc = np.ascontiguousarray(per_class, np.float32)
+    S = np.ascontiguousarray(seeds, np.int32)
+    return lib().oift_gpu_run(int(cost_mode), img.ctypes.data, h_labels.ctypes.data, img.size, xsize, ysize, zsize,
+                              pc.ctypes.data, pc.size - 1, S.ctypes.data, S.size)

@@ -13,6 +13,7 @@
 // one HBM read and one write per voxel.  An odd pass count ends with the single-pass form
 // of the same kernel.  This file is compiled without FMA contraction: every product and
 // sum rounds as the reference's FP32 code does, so the result is bit-exact.
+#include <algorithm>
 #include <cmath>
 
 #include "ctx.hpp"
@@ -32,7 +33,7 @@ struct BlurArgs {
     unsigned ntx, nty, nzc;   // tiles in x and y, plane chunks in z
     int zc;                   // planes per chunk
     int xcd;                  // deal consecutive tiles to one XCD
-    float k0, k1;             // normalised centre and face taps
+    float k0, kz, ky, kx;     // normalised taps: centre, z, y and x neighbours
 };
 
 // ROUND(x) of gft_common.h:120 is (x < 0) ? (int)(x - 0.5) : (int)(x + 0.5) with a double
@@ -51,19 +52,19 @@ __device__ __forceinline__ float gft_round(float conv)
 // One voxel: FP32 products and sums in the reference's tap order (centre, -z, -y, -x, +x,
 // +y, +z).  Values outside the volume are staged as +0.0f, and (float)q * k with q = +0.0f
 // adds +0.0f, which leaves the running sum unchanged (it starts at +0.0f and is never
-// -0.0f), exactly as skipping the tap does.
-__device__ __forceinline__ float blur_voxel(float k0, float k1, float c, float zm, float ym, float xm, float xp,
+// -0.0f), exactly as skipping the tap does.  The taps are positive.
+__device__ __forceinline__ float blur_voxel(const BlurArgs &a, float c, float zm, float ym, float xm, float xp,
                                             float yp, float zp)
 {
     // the reference starts from conv = 0.0f; 0.0f + c * k0 == c * k0 because c, an
     // integer converted to float, is never -0.0f
-    float conv = c * k0;
-    conv += zm * k1;
-    conv += ym * k1;
-    conv += xm * k1;
-    conv += xp * k1;
-    conv += yp * k1;
-    conv += zp * k1;
+    float conv = c * a.k0;
+    conv += zm * a.kz;
+    conv += ym * a.ky;
+    conv += xm * a.kx;
+    conv += xp * a.kx;
+    conv += yp * a.ky;
+    conv += zp * a.kz;
     return gft_round(conv);
 }
 
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(THREADS) void k_blur(const IT *__restrict__ in, IT 
 #pragma unroll
         for (int r = 0; r < NOUT; r++) {
             const int s = (ty0 + r * ROWS + H) * W + tx + H;
-            const float v = blur_voxel(a.k0, a.k1, pc[s], pm[s], pc[s - W], pc[s - 1], pc[s + 1], pc[s + W], pp[s]);
+            const float v = blur_voxel(a, pc[s], pm[s], pc[s - W], pc[s - 1], pc[s + 1], pc[s + W], pp[s]);
             if ((omask >> r) & 1u) o[(uint32_t)(r * ROWS) * (uint32_t)a.xs] = (IT)(int32_t)v;
         }
     };
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(THREADS) void k_blur(const IT *__restrict__ in, IT 
                 if (r < NP1 - 1 || idx < NP) {
                     const int s = p1s[r];
                     const float v =
-                        blur_voxel(a.k0, a.k1, cc[s], cm[s], cc[s - IX], cc[s - 1], cc[s + 1], cc[s + IX], cp[s]);
+                        blur_voxel(a, cc[s], cm[s], cc[s - IX], cc[s - 1], cc[s + 1], cc[s + IX], cp[s]);
                     dst[idx] = ((m >> r) & 1u) ? v : 0.0f;
                 }
             }
@@ -217,20 +218,28 @@ __global__ __launch_bounds__(THREADS) void k_blur(const IT *__restrict__ in, IT 
     }
 }
 
-// Kernel3::SphericalGaussian(Spheric(1.0), R = 1, s = 10, f = 1) + Normalize, computed on
-// the host exactly as the reference's host code does (exp of a double, float sum in tap order).
-void gauss_taps(float &k0, float &k1)
+// Kernel3::SphericalGaussian(scn, R = 1, s = 10, f = 1) + Normalize over Spheric(1.0),
+// computed on the host exactly as the reference's host code does (gft_filtering3.cpp:106-126,
+// 72-83): displacements scaled by spacing / min spacing in float, exp of a float, float
+// sum in tap order, float division.
+void gauss_taps(const Dims &d, BlurArgs &a)
 {
     const float R = 1.0f, s = 10.0f, f = 1.0f, R2 = R * R;
+    const float dmin = std::min(d.sx, std::min(d.sy, d.sz));
     float val[7];
     for (int i = 0; i < 7; i++) {
-        const float r2 = i == 0 ? 0.0f : 1.0f;
-        val[i] = (float)((double)s * std::exp((double)(-f * (r2 / R2))));
+        const float d_x = ((float)Adjacency<7>::dx(i) * d.sx) / dmin;
+        const float d_y = ((float)Adjacency<7>::dy(i) * d.sy) / dmin;
+        const float d_z = ((float)Adjacency<7>::dz(i) * d.sz) / dmin;
+        const float r2 = d_x * d_x + d_y * d_y + d_z * d_z;
+        val[i] = s * std::exp(-f * (r2 / R2));   // the float overload, as the reference's exp(float)
     }
     float wt = 0.0f;
     for (int i = 0; i < 7; i++) wt += val[i];
-    k0 = val[0] / wt;
-    k1 = val[1] / wt;
+    a.k0 = val[0] / wt;
+    a.kz = val[1] / wt;   // -z (the +z tap, i = 6, has the same value)
+    a.ky = val[2] / wt;
+    a.kx = val[3] / wt;
 }
 
 template <typename IT>
@@ -265,7 +274,7 @@ int run_gaussian_blur(roift_ctx &c, const Dims &d, const void *d_in, int dtype, 
     a.xs = d.xs;
     a.ys = d.ys;
     a.zs = d.zs;
-    gauss_taps(a.k0, a.k1);
+    gauss_taps(d, a);
     a.ntx = (unsigned)((d.xs + BX - 1) / BX);
     a.nty = (unsigned)((d.ys + BY - 1) / BY);
     // planes per workgroup: ROIFT_PARAM_BLUR_PLANES, else enough chunks for ~1024 workgroups

@@ -1,6 +1,9 @@
 // capi.hip -- the extern "C" boundary of libroift.so (declared in include/roift.h).
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <set>
+#include <vector>
 
 #include "ctx.hpp"
 
@@ -26,6 +29,11 @@ Dims dims_of(const roift_problem *pb)
     ROIFT_REQUIRE(pb->xsize > 0 && pb->ysize > 0 && pb->zsize > 0, ROIFT_EINVAL, "non-positive volume size");
     Dims d{pb->xsize, pb->ysize, pb->zsize, (long long)pb->xsize * pb->ysize * pb->zsize};
     ROIFT_REQUIRE(d.n < (1ll << 31), ROIFT_EINVAL, "volume larger than 2^31-1 voxels");
+    ROIFT_REQUIRE(std::isfinite(pb->dx) && std::isfinite(pb->dy) && std::isfinite(pb->dz), ROIFT_EINVAL,
+                  "non-finite voxel spacing");
+    d.sx = pb->dx != 0.0f ? pb->dx : 1.0f;
+    d.sy = pb->dy != 0.0f ? pb->dy : 1.0f;
+    d.sz = pb->dz != 0.0f ? pb->dz : 1.0f;
     return d;
 }
 
@@ -128,8 +136,38 @@ Weights weights_into_ctx(roift_ctx &c, const Dims &d, int nadj, const void *img,
         w.wb = buf<uint32_t>(c, "pipe.wb32", slots);
         w.info = build_weights(c, d, nadj, img, idt, P, w.wa, w.wb, ROIFT_U32, gm);
     }
-    ROIFT_REQUIRE(w.info.minI >= 0, ROIFT_EINVAL, "image has negative values (shift it to >= 0 first)");
+    // the image form's weights |I_p - I_q| stay within the reference's queue only for I >= 0
+    // (oiftrelax.cpp:263-268 shifts it); the W forms' weights do not depend on its sign
+    const bool w_base = gm && !gm->G;
+    ROIFT_REQUIRE(w_base || w.info.minI >= 0, ROIFT_EINVAL, "image has negative values (shift it to >= 0 first)");
     return w;
+}
+
+// The per-voxel weight forms OIFT(W, ...) / OIFT(Wx, Wy, Wz, ...): W >= 0 (the reference's
+// bucket queue has Wmax + 2 buckets, Wmax = 2 max W (1 + |per|/100), gft_ift.cpp:1101-1103,
+// 1201-1206), and W_p + W_q (1 + |per|/100) within int32, as the reference's int arithmetic.
+GradModel w_model(roift_ctx &c, const Dims &d, int nadj, const int32_t *W, const int32_t *Wx, const int32_t *Wy,
+                  const int32_t *Wz, float per)
+{
+    GradModel gm;
+    const bool axes = Wx || Wy || Wz;
+    ROIFT_REQUIRE(axes ? (Wx && Wy && Wz && !W) : (W != nullptr), ROIFT_EINVAL,
+                  "give either W or all of Wx, Wy, Wz");
+    ROIFT_REQUIRE(!axes || nadj == 7, ROIFT_EINVAL, "the per-axis form is defined on Spheric(1.0) only");
+    long long wmax = 0;
+    for (const int32_t *w : {W, Wx, Wy, Wz}) {
+        if (!w) continue;
+        const WeightInfo e = image_extrema(c, d, w, ROIFT_I32);
+        ROIFT_REQUIRE(e.minI >= 0, ROIFT_EINVAL, "per-voxel weights must be >= 0");
+        wmax = std::max<long long>(wmax, e.maxI);
+    }
+    ROIFT_REQUIRE((double)(2 * wmax) * (1.0 + std::fabs((double)per) / 100.0) < 2147483647.0, ROIFT_EUNSUPPORTED,
+                  "per-voxel weights too large for int32 arc weights");
+    gm.W = W;
+    gm.Wx = Wx;
+    gm.Wy = Wy;
+    gm.Wz = Wz;
+    return gm;
 }
 
 }  // namespace
@@ -246,6 +284,42 @@ ROIFT_API int roift_oift_fmax_dev(roift_ctx *ctx, const roift_problem *pb, const
         const Polarity po = polarity_of(pb);
         ev_record(*ctx, 0);
         const Weights w = weights_into_ctx(*ctx, d, nadj, d_image, image_dtype, po.P);
+        ev_record(*ctx, 1);
+        const int lv = run_oift(*ctx, d, nadj, w.wa, w.wb, w.dtype, po.lc, d_seeds, n_seeds, d_label, d_cost);
+        ev_record(*ctx, 2);
+        ctx->timings[0] = ev_ms(*ctx, 0, 1);
+        ctx->timings[1] = ev_ms(*ctx, 1, 2);
+        return lv >= 0 ? ROIFT_OK : lv;
+    });
+}
+
+ROIFT_API int roift_weights_w_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                                  const int32_t *d_W, const int32_t *d_Wx, const int32_t *d_Wy, const int32_t *d_Wz,
+                                  void *d_wa, void *d_wb, int weight_dtype)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        const int nadj = nadj_of(pb);
+        ROIFT_REQUIRE(!pb->use_per_class, ROIFT_EINVAL, "the weight builder takes a single polarity (per)");
+        ROIFT_REQUIRE(std::fabs(pb->per) <= 100.0f, ROIFT_EINVAL, "|per| > 100");
+        const GradModel gm = w_model(*ctx, d, nadj, d_W, d_Wx, d_Wy, d_Wz, pb->per);
+        build_weights(*ctx, d, nadj, d_image, image_dtype, pb->per, d_wa, d_wb, weight_dtype, &gm);
+        return ROIFT_OK;
+    });
+}
+
+ROIFT_API int roift_oift_w_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
+                               const int32_t *d_W, const int32_t *d_Wx, const int32_t *d_Wy, const int32_t *d_Wz,
+                               const int32_t *d_seeds, int n_seeds, int32_t *d_label, int32_t *d_cost)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        const int nadj = nadj_of(pb);
+        ROIFT_REQUIRE(!pb->use_per_class, ROIFT_EINVAL, "OIFT(W, ...) takes a single polarity (per)");
+        const Polarity po = polarity_of(pb);   // label > 0 -> +per, label 0 -> -per (gft_ift.cpp:1140-1143)
+        const GradModel gm = w_model(*ctx, d, nadj, d_W, d_Wx, d_Wy, d_Wz, po.P);
+        ev_record(*ctx, 0);
+        const Weights w = weights_into_ctx(*ctx, d, nadj, d_image, image_dtype, po.P, &gm);
         ev_record(*ctx, 1);
         const int lv = run_oift(*ctx, d, nadj, w.wa, w.wb, w.dtype, po.lc, d_seeds, n_seeds, d_label, d_cost);
         ev_record(*ctx, 2);
@@ -485,6 +559,37 @@ ROIFT_API int roift_oift_fmax(roift_ctx *ctx, const roift_problem *pb, const int
     });
 }
 
+ROIFT_API int roift_oift_w(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image, const int32_t *h_W,
+                           const int32_t *h_Wx, const int32_t *h_Wy, const int32_t *h_Wz, const int32_t *h_seeds,
+                           int n_seeds, int32_t *h_label, int32_t *h_cost)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        auto up = [&](const int32_t *h, const char *name) -> int32_t * {
+            if (!h) return nullptr;
+            int32_t *dv = buf<int32_t>(*ctx, name, d.n);
+            ROIFT_HIP_CHECK(hipMemcpyAsync(dv, h, 4 * d.n, hipMemcpyHostToDevice, ctx->stream));
+            return dv;
+        };
+        int32_t *img = up(h_image, "host.img");
+        int32_t *W = up(h_W, "host.W");
+        int32_t *Wx = up(h_Wx, "host.Wx");
+        int32_t *Wy = up(h_Wy, "host.Wy");
+        int32_t *Wz = up(h_Wz, "host.Wz");
+        int32_t *lab = up(h_label, "host.lab");
+        int32_t *cost = h_cost ? buf<int32_t>(*ctx, "host.cost", d.n) : nullptr;
+        int32_t *S = buf<int32_t>(*ctx, "host.seeds", n_seeds > 0 ? n_seeds : 1);
+        if (n_seeds > 0)
+            ROIFT_HIP_CHECK(hipMemcpyAsync(S, h_seeds, 4 * (size_t)n_seeds, hipMemcpyHostToDevice, ctx->stream));
+        int rc = roift_oift_w_dev(ctx, pb, img, ROIFT_I32, W, Wx, Wy, Wz, S, n_seeds, lab, cost);
+        if (rc != ROIFT_OK) return rc;
+        ROIFT_HIP_CHECK(hipMemcpyAsync(h_label, lab, 4 * d.n, hipMemcpyDeviceToHost, ctx->stream));
+        if (h_cost) ROIFT_HIP_CHECK(hipMemcpyAsync(h_cost, cost, 4 * d.n, hipMemcpyDeviceToHost, ctx->stream));
+        ROIFT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return ROIFT_OK;
+    });
+}
+
 ROIFT_API int roift_relax(roift_ctx *ctx, const roift_problem *pb, const int32_t *h_image, const int32_t *h_seeds,
                           int n_seeds, int32_t *h_label, int niter, float *h_memb)
 {
@@ -632,21 +737,72 @@ ROIFT_API int roift_volume_read_dev(roift_ctx *ctx, const char *path, void *d_ds
     return guarded(ctx, [&]() -> int { return volume_read_dev(*ctx, path, d_dst, dst_dtype, flags, h_shift, info); });
 }
 
-ROIFT_API int oift_fmax_gpu_run(const int *h_image, int *h_labels, int n, int xsize, int ysize, int zsize,
-                                const float *h_per_class, int max_label, const int *h_seeds, int n_seeds)
+}  // extern "C"
+
+namespace {
+
+// The input contract the reference's two GPU entry points share (gpu/oift_fmax_gpu.cu:171-185,
+// gpu/oift_gpu.cu:192-211), over the exact OIFT_Multi_PerClass of this library:
+//  - every voxel that is not a listed seed starts unlabelled, whatever h_labels holds;
+//  - a seed's label is h_labels[seed], negative -> 0, and with clamp_high also > max_label -> 0;
+//  - seeds outside [0, n) are skipped, and a repeated seed counts once (at its first position,
+//    which is its place in the FIFO order);
+//  - on return every voxel holds its label, 0 where no seed reached it (the reference unpacks
+//    label 0 from the untouched (COST_INF, 0) words).
+// With no valid seed, oift_gpu_run leaves h_labels as it is and returns 0 (oift_gpu.cu:207-210);
+// oift_fmax_gpu_run writes label 0 everywhere and returns 0, as its untouched words unpack.
+// Returns the number of level steps (the reference returns its wave count), or -1.
+int legacy_run(bool gpu_run_rules, const int *h_image, int *h_labels, int n, int xsize, int ysize, int zsize,
+               const float *h_per_class, int max_label, const int *h_seeds, int n_seeds)
 {
-    if (!h_image || !h_labels || !h_per_class || max_label < 0 || n != xsize * ysize * zsize) return -1;
-    if (n_seeds <= 0) return 0;  // nothing to propagate
+    if (!h_image || !h_labels || !h_per_class || max_label < 0 || xsize <= 0 || ysize <= 0 || zsize <= 0 ||
+        (long long)n != (long long)xsize * ysize * zsize || n_seeds < 0 || (n_seeds > 0 && !h_seeds))
+        return -1;
+    std::vector<int32_t> lab((size_t)n, -1), seeds;
+    seeds.reserve((size_t)n_seeds);
+    for (int i = 0; i < n_seeds; i++) {
+        const int p = h_seeds[i];
+        if (p < 0 || p >= n || lab[p] != -1) continue;
+        int lb = h_labels[p];
+        if (lb < 0) lb = 0;
+        if (gpu_run_rules && lb > max_label) lb = 0;
+        lab[p] = lb;
+        seeds.push_back(p);
+    }
+    if (seeds.empty()) {
+        if (!gpu_run_rules) std::fill(h_labels, h_labels + n, 0);
+        return 0;
+    }
     roift_ctx *ctx = nullptr;
     if (roift_create(&ctx, 0) != ROIFT_OK) return -1;
     roift_problem pb{xsize, ysize, zsize, 1.0f, 1, 0.0f, h_per_class, max_label};
-    // the reference maps negative seed labels to background (gpu/oift_fmax_gpu.cu:178-181)
-    for (int i = 0; i < n_seeds; i++)
-        if (h_seeds[i] >= 0 && h_seeds[i] < n && h_labels[h_seeds[i]] < 0) h_labels[h_seeds[i]] = 0;
-    int rc = roift_oift_fmax(ctx, &pb, h_image, h_seeds, n_seeds, h_labels, nullptr);
+    const int rc = roift_oift_fmax(ctx, &pb, h_image, seeds.data(), (int)seeds.size(), lab.data(), nullptr);
     const int steps = (int)ctx->timings[4];
+    if (rc != ROIFT_OK) std::fprintf(stderr, "roift: %s\n", roift_last_error(ctx));
     roift_destroy(ctx);
-    return rc == ROIFT_OK ? steps : -1;
+    if (rc != ROIFT_OK) return -1;
+    for (int p = 0; p < n; p++) h_labels[p] = lab[p] < 0 ? 0 : lab[p];
+    return steps;
+}
+
+}  // namespace
+
+extern "C" {
+
+ROIFT_API int oift_fmax_gpu_run(const int *h_image, int *h_labels, int n, int xsize, int ysize, int zsize,
+                                const float *h_per_class, int max_label, const int *h_seeds, int n_seeds)
+{
+    return legacy_run(false, h_image, h_labels, n, xsize, ysize, zsize, h_per_class, max_label, h_seeds, n_seeds);
+}
+
+ROIFT_API int oift_gpu_run(int cost_mode, const int *h_image, int *h_labels, int n, int xsize, int ysize, int zsize,
+                           const float *h_per_class, int max_label, const int *h_seeds, int n_seeds)
+{
+    if (cost_mode != 0) {   // 1 = additive path cost: not part of the f_max path this library implements
+        std::fprintf(stderr, "roift: oift_gpu_run cost_mode %d is not supported (f_max, 0, only)\n", cost_mode);
+        return -1;
+    }
+    return legacy_run(true, h_image, h_labels, n, xsize, ysize, zsize, h_per_class, max_label, h_seeds, n_seeds);
 }
 
 }  // extern "C"

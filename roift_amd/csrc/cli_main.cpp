@@ -140,7 +140,9 @@ int run(Cli &cl, int argc, char **argv)
     int32_t *d_seeds = cl.dmalloc<int32_t>((size_t)(count + face_cap) * 4);
     hip_check("copy labels", hipMemcpy(d_label, h_label, (size_t)n * 4, hipMemcpyHostToDevice));
     if (count > 0) hip_check("copy seeds", hipMemcpy(d_seeds, seeds.data(), (size_t)count * 4, hipMemcpyHostToDevice));
-    roift_problem pb{info.xsize, info.ysize, info.zsize, 1.0f, 0, 0.0f, nullptr, -1};
+    // the scene's spacing reaches the blur taps and the relaxation's Dpq, as the reference's
+    // fscn carries it (gft_filtering3.cpp:390-392)
+    roift_problem pb{info.xsize, info.ysize, info.zsize, 1.0f, 0, 0.0f, nullptr, -1, info.dx, info.dy, info.dz};
     int total = count;
     if (boundary_stride > 0) {   // appended after the file's seeds, in raster order (oiftrelax.cpp:357-364)
         int added = 0;

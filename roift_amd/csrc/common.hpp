@@ -40,6 +40,9 @@ template <> struct Adjacency<27> {
 struct Dims {
     int xs, ys, zs;
     long long n;
+    // voxel spacing (gft's scn->dx/dy/dz): only the relaxation's Dpq and the Gaussian blur's
+    // taps read it (gft_ift.cpp:8723-8731, gft_filtering3.cpp:106-126)
+    float sx = 1.0f, sy = 1.0f, sz = 1.0f;
 };
 
 // Division by an invariant d for 0 <= n < 2^31 with one mul-hi and a shift

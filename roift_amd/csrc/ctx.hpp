@@ -87,10 +87,14 @@ struct WeightInfo {
 };
 
 // Entry points implemented in the .hip translation units.
-// Optional gradient-weight model of the builder (OIFT_Multi_GradWeight): int32 G, blend alpha.
+// Optional base of the builder's arc weight in place of |I_p - I_q|: the gradient-weight
+// model (OIFT_Multi_GradWeight: int32 G, blend alpha), a per-voxel weight W (OIFT(W, ...):
+// W_p + W_q) or per-axis weights Wx, Wy, Wz (OIFT(Wx, Wy, Wz, ...), Spheric(1.0) only).
 struct GradModel {
     const int32_t *G = nullptr;
     float alpha = 0.5f;
+    const int32_t *W = nullptr;
+    const int32_t *Wx = nullptr, *Wy = nullptr, *Wz = nullptr;
 };
 
 WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,

@@ -1,7 +1,7 @@
 // gft_dropin.cpp -- gft-signature entry points over the C-ABI (libroift_gft.so).
 //
 // Same behaviour contract as the reference functions they replace
-// (gft_ift.cpp:911-1083, 8655-9187; GaussianBlur, gft_filtering3.cpp:385-397): `label` is modified in place; a NULL S
+// (gft_ift.cpp:828-1271, 8498-9498; GaussianBlur, gft_filtering3.cpp:385-397): `label` is modified in place; a NULL S
 // seeds every labelled voxel in raster order (gft_ift.cpp:944-948); where the
 // reference would call gft::Error (exit(-1)), this prints the library's error
 // and exits with the same status, since these void signatures cannot report it.
@@ -60,7 +60,8 @@ void check(int rc)
 roift_problem problem(const gft::sAdjRel3 *A, const gft::sScene32 *scn, int use_pc, float per, const float *pc,
                       int max_label)
 {
-    return roift_problem{scn->xsize, scn->ysize, scn->zsize, radius_of(A), use_pc, per, pc, max_label};
+    return roift_problem{scn->xsize, scn->ysize, scn->zsize, radius_of(A), use_pc, per, pc, max_label,
+                         scn->dx, scn->dy, scn->dz};
 }
 
 }  // namespace
@@ -81,6 +82,37 @@ __attribute__((visibility("default"))) void OIFT_Multi_PerClass(sAdjRel3 *A, sSc
     const std::vector<int> s = seeds_of(S, label);
     const roift_problem pb = problem(A, scn, 1, 0.0f, per_class, max_label);
     check(roift_oift_fmax(context(), &pb, scn->data, s.data(), (int)s.size(), label->data, nullptr));
+}
+
+// OIFT(A, scn, per, S, label) (gft_ift.cpp:828-908): the two-label outer cut polarises by
+// label > 0 ? per : -per, which for the labels >= 0 a popped voxel carries is OIFT_Multi's rule.
+__attribute__((visibility("default"))) void OIFT(sAdjRel3 *A, sScene32 *scn, float per, int *S, sScene32 *label)
+{
+    const std::vector<int> s = seeds_of(S, label);
+    const roift_problem pb = problem(A, scn, 0, per, nullptr, -1);
+    check(roift_oift_fmax(context(), &pb, scn->data, s.data(), (int)s.size(), label->data, nullptr));
+}
+
+// OIFT(W, A, scn, per, S, label) (gft_ift.cpp:1086-1167): base W_p + W_q.
+__attribute__((visibility("default"))) void OIFT(sScene32 *W, sAdjRel3 *A, sScene32 *scn, float per, int *S,
+                                                 sScene32 *label)
+{
+    const std::vector<int> s = seeds_of(S, label);
+    const roift_problem pb = problem(A, scn, 0, per, nullptr, -1);
+    check(roift_oift_w(context(), &pb, scn->data, W->data, nullptr, nullptr, nullptr, s.data(), (int)s.size(),
+                       label->data, nullptr));
+}
+
+// OIFT(Wx, Wy, Wz, scn, per, S, label) (gft_ift.cpp:1170-1271): per-axis bases on Spheric(1.0).
+__attribute__((visibility("default"))) void OIFT(sScene32 *Wx, sScene32 *Wy, sScene32 *Wz, sScene32 *scn, float per,
+                                                 int *S, sScene32 *label)
+{
+    static gft::AdjRel3::sDisplacement3 d6[7];
+    static sAdjRel3 six{d6, 7};   // gft_ift.cpp:1191
+    const std::vector<int> s = seeds_of(S, label);
+    const roift_problem pb = problem(&six, scn, 0, per, nullptr, -1);
+    check(roift_oift_w(context(), &pb, scn->data, nullptr, Wx->data, Wy->data, Wz->data, s.data(), (int)s.size(),
+                       label->data, nullptr));
 }
 
 __attribute__((visibility("default"))) void ORelax_1_Multi(sAdjRel3 *A, sScene32 *scn, float per, int *S,
@@ -150,7 +182,7 @@ __attribute__((visibility("default"))) sScene32 *GaussianBlur(sScene32 *scn)
     blur->dx = scn->dx;
     blur->dy = scn->dy;
     blur->dz = scn->dz;
-    const roift_problem pb{scn->xsize, scn->ysize, scn->zsize, 1.0f, 0, 0.0f, nullptr, -1};
+    const roift_problem pb{scn->xsize, scn->ysize, scn->zsize, 1.0f, 0, 0.0f, nullptr, -1, scn->dx, scn->dy, scn->dz};
     check(roift_gaussian_blur(context(), &pb, scn->data, blur->data, 1));
     return blur;
 }

@@ -28,6 +28,27 @@ namespace {
 constexpr uint8_t NOSEED = 255;
 constexpr int NC_MAX = 8;
 
+// Dpq[k] = sqrtf(ax^2 dx^2 + ay^2 dy^2 + az^2 dz^2) / min(dx, dy, dz) for the Spheric
+// displacements, in the reference's float operation order (gft_ift.cpp:8723-8731; the
+// same in ORelax_1, :8515-8522, and in the experiments' relaxations).  Returns whether any
+// Dpq differs from 1 (never for 6-adjacency at isotropic spacing).
+int spheric_dpq(const Dims &d, int nadj, float *Dpq)
+{
+    const float dmin = std::min(d.sx, std::min(d.sy, d.sz));
+    int div = 0;
+    for (int k = 0; k < 27; k++) Dpq[k] = 1.0f;
+    for (int k = 1; k < nadj; k++) {
+        const int ax = nadj == 7 ? Adjacency<7>::dx(k) : Adjacency<27>::dx(k);
+        const int ay = nadj == 7 ? Adjacency<7>::dy(k) : Adjacency<27>::dy(k);
+        const int az = nadj == 7 ? Adjacency<7>::dz(k) : Adjacency<27>::dz(k);
+        Dpq[k] = sqrtf((float)(ax * ax) * d.sx * d.sx + (float)(ay * ay) * d.sy * d.sy +
+                       (float)(az * az) * d.sz * d.sz) /
+                 dmin;
+        if (Dpq[k] != 1.0f) div = 1;
+    }
+    return div;
+}
+
 struct RelaxArgs {
     Dims d;
     float per;          // OIFT_Multi form
@@ -35,6 +56,7 @@ struct RelaxArgs {
     float Wmax;
     float tA, tB;       // s/100 = tA * p_bg(q) + tB  (z-marching form)
     float Dpq[27];
+    int dpq_div;        // some Dpq[k] != 1: divide (dividing by 1.0f is exact, so it is skipped otherwise)
     int use_pc;
     int nc;
     int bg;
@@ -74,7 +96,7 @@ __device__ inline float model_weight(const RelaxArgs &a, int k, int Ip, int Iq, 
     w = w * w;
     w = w * w;
     w = w * w;
-    if (NADJ != 7) w /= a.Dpq[k];
+    if (a.dpq_div) w /= a.Dpq[k];
     return w;
 }
 
@@ -275,7 +297,9 @@ struct MarchVol {
     float wmax;
 };
 
-template <bool BATCH>
+// DPQ: anisotropic spacing, each neighbour's weight divided by its axis's Dpq (1 = -z, 2 = -y,
+// 3 = -x, 4 = +x, 5 = +y, 6 = +z in Spheric order) after the eighth power, as the reference does.
+template <bool BATCH, bool DPQ>
 __global__ __launch_bounds__(MX *MY_MAX) void k_relax2_march(const int16_t *__restrict__ img, RelaxArgs a,
                                                           const MarchTile *__restrict__ tiles,
                                                           const MarchVol *__restrict__ vols,
@@ -355,24 +379,30 @@ __global__ __launch_bounds__(MX *MY_MAX) void k_relax2_march(const int16_t *__re
             float sw = 0.0f, acc = 0.0f, w;
             if (hm) {
                 w = relax_w8(Ic - Im, fm, a);
+                if constexpr (DPQ) w /= a.Dpq[1];
                 sw += w;
                 acc = fmaf(w, fm, acc);
             }
             w = keep(relax_w8(Ic - (float)(rym & 0x7FFF), fym, a), kym);
+            if constexpr (DPQ) w /= a.Dpq[2];
             sw += w;
             acc = fmaf(w, fym, acc);
             w = keep(relax_w8(Ic - (float)(rxm & 0x7FFF), fxm, a), kxm);
+            if constexpr (DPQ) w /= a.Dpq[3];
             sw += w;
             acc = fmaf(w, fxm, acc);
             w = keep(relax_w8(Ic - (float)(rxp & 0x7FFF), fxp, a), kxp);
+            if constexpr (DPQ) w /= a.Dpq[4];
             sw += w;
             acc = fmaf(w, fxp, acc);
             w = keep(relax_w8(Ic - (float)(ryp & 0x7FFF), fyp, a), kyp);
+            if constexpr (DPQ) w /= a.Dpq[5];
             sw += w;
             acc = fmaf(w, fyp, acc);
             if (hp) {
                 const float In = (float)(rn & 0x7FFF);
                 w = relax_w8(Ic - In, fn, a);
+                if constexpr (DPQ) w /= a.Dpq[6];
                 sw += w;
                 acc = fmaf(w, fn, acc);
             }
@@ -433,6 +463,7 @@ struct BinArgs {
     float per;
     float Wmax;   // (float) of the reference's int Wmax
     float Dpq[27];
+    int dpq_div;  // as RelaxArgs::dpq_div
     const int32_t *W;
     const int32_t *Wa[3];   // x, y, z
 };
@@ -476,7 +507,7 @@ __global__ __launch_bounds__(256) void k_relax_bin(const IT *__restrict__ img, B
                 w = w * w;
                 w = w * w;
                 w = w * w;
-                if (BM == BIN_IMAGE && NADJ != 7) w /= a.Dpq[k];
+                if (BM == BIN_IMAGE && a.dpq_div) w /= a.Dpq[k];
                 sw += w;
                 acc += w * fq;
             }
@@ -625,13 +656,7 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
     a.G = rm.G;
     a.alpha = rm.alpha;
     a.sigma2 = rm.sigma * rm.sigma;
-    for (int k = 0; k < 27; k++) a.Dpq[k] = 1.0f;
-    if (nadj == 27)
-        for (int k = 1; k < 27; k++) {
-            const int ax = Adjacency<27>::dx(k), ay = Adjacency<27>::dy(k), az = Adjacency<27>::dz(k);
-            a.Dpq[k] = sqrtf((float)(ax * ax) * 1.0f * 1.0f + (float)(ay * ay) * 1.0f * 1.0f +
-                             (float)(az * az) * 1.0f * 1.0f) / 1.0f;
-        }
+    a.dpq_div = spheric_dpq(d, nadj, a.Dpq);
     for (int k = 0; k < nc; k++) a.cls_label[k] = cls[k];
     if (nc == 2) {
         if (rs.use_per_class) {
@@ -689,8 +714,12 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
     for (int it = 0; it < niter; it++) {
         int32_t *lo = (it == niter - 1) ? d_label : nullptr;
         if (march) {
-            hipLaunchKernelGGL(k_relax2_march<false>, mgrid, dim3(MX, a.my), 0, st, pimg, a, nullptr, nullptr, fin, fout,
-                               lo);
+            if (a.dpq_div)
+                hipLaunchKernelGGL((k_relax2_march<false, true>), mgrid, dim3(MX, a.my), 0, st, pimg, a, nullptr,
+                                   nullptr, fin, fout, lo);
+            else
+                hipLaunchKernelGGL((k_relax2_march<false, false>), mgrid, dim3(MX, a.my), 0, st, pimg, a, nullptr,
+                                   nullptr, fin, fout, lo);
             ROIFT_HIP_CHECK(hipGetLastError());
         } else if (nadj == 7) {
             if (i16) launch_iter<7, int16_t>(c, d_image, a, rm.model, fin, fout, seedcls, lo);
@@ -748,13 +777,7 @@ int run_relax_binary(roift_ctx &c, const Dims &d, int nadj, const void *d_image,
     a.d = d;
     a.per = per;
     a.Wmax = (float)(int)((double)base_max * (1.0 + (double)std::fabs(per) / 100.0));
-    for (int k = 0; k < 27; k++) a.Dpq[k] = 1.0f;
-    if (nadj == 27)
-        for (int k = 1; k < 27; k++) {
-            const int ax = Adjacency<27>::dx(k), ay = Adjacency<27>::dy(k), az = Adjacency<27>::dz(k);
-            a.Dpq[k] = sqrtf((float)(ax * ax) * 1.0f * 1.0f + (float)(ay * ay) * 1.0f * 1.0f +
-                             (float)(az * az) * 1.0f * 1.0f) / 1.0f;
-        }
+    a.dpq_div = spheric_dpq(d, nadj, a.Dpq);
     a.W = W;
     a.Wa[0] = Wx;
     a.Wa[1] = Wy;
@@ -860,18 +883,28 @@ int run_relax_batch(roift_ctx &c, const std::vector<BatchVolume> &vols, const vo
         float *fin = f0, *fout = f1;
         for (int it = 0; it < niter; it++) {
             int32_t *lo = (it == niter - 1) ? d_label : nullptr;
-            hipLaunchKernelGGL(k_relax2_march<true>, dim3(nt), dim3(MX, MY), 0, st, pimg, a, dtile, dvol, fin, fout,
-                               lo);
+            hipLaunchKernelGGL((k_relax2_march<true, false>), dim3(nt), dim3(MX, MY), 0, st, pimg, a, dtile, dvol,
+                               fin, fout, lo);
             std::swap(fin, fout);
         }
     };
     ev_record(c, 12);
     bool graphed = false;
     if (use_graph && niter > 0 && st != nullptr) {
-        // the iteration launches as one HIP graph, re-instantiated when the batch changes
-        char key[256];
-        std::snprintf(key, sizeof(key), "%p %p %p %p %p %p %u %d %lld %a", (void *)pimg, (void *)dtile,
-                      (void *)dvol, (void *)f0, (void *)f1, (void *)d_label, nt, niter, n, (double)per);
+        // the iteration launches as one HIP graph, re-instantiated when anything a captured
+        // launch holds by value changes: the buffers, the grid, the iteration count and every
+        // byte of the kernel's RelaxArgs (object label, polarity, sizes; `a` is
+        // value-initialised, so its padding is zero and the bytes are deterministic)
+        char head[160];
+        std::snprintf(head, sizeof(head), "%p %p %p %p %p %p %u %d|", (void *)pimg, (void *)dtile, (void *)dvol,
+                      (void *)f0, (void *)f1, (void *)d_label, nt, niter);
+        std::string key(head);
+        const unsigned char *ab = reinterpret_cast<const unsigned char *>(&a);
+        static const char hx[] = "0123456789abcdef";
+        for (size_t i = 0; i < sizeof(a); i++) {
+            key.push_back(hx[ab[i] >> 4]);
+            key.push_back(hx[ab[i] & 15]);
+        }
         if (!c.relax_graph || c.relax_graph_key != key) {
             if (c.relax_graph) {
                 ROIFT_HIP_CHECK(hipGraphExecDestroy(c.relax_graph));

@@ -166,6 +166,9 @@ void read_header(const std::string &path, Header &h)
         h.xs = std::max<int>(1, dim[1]);
         h.ys = std::max<int>(1, dim[2]);
         h.zs = std::max<int>(1, dim[3]);
+        // nifti2_io sets a zero or non-finite grid spacing of a used axis to 1.0 (nifti2_io.c:3670-3673)
+        for (int i = 1; i <= dim[0] && i < 8; i++)
+            if (pix[i] == 0.0f || !std::isfinite(pix[i])) pix[i] = 1.0f;
         h.dx = pix[1];
         h.dy = pix[2];
         h.dz = pix[3];

@@ -17,17 +17,25 @@ namespace roift {
 // Block shape of the weight builder: 64 x 4 (x, y) voxels, marching WZ planes in z.
 constexpr int WX = 64, WY = 4, WZ = 16;
 
-// GRAD = false: OIFT_Multi's weight, |I_p-I_q| * (1 +- per/100) in double, truncated.
-// GRAD = true:  OIFT_Multi_GradWeight's weight (exp_gradient_weight.cpp:244-254), all in
-// float: wf = alpha|I_p-I_q| + (1-alpha)(G_p+G_q)*0.5, (int)(wf (1 +- per/100)), (int)wf when
-// I_p == I_q, clamped at 0.  wf is symmetric in p and q, so the two-values-per-edge
-// layout holds for it too.  This file is compiled without FMA contraction.
+// Base of the arc weight, to which the polarity factor (1 +- per/100) applies by the order
+// of I_p and I_q (unchanged when they are equal):
+// WB_IMAGE: OIFT_Multi / OIFT, |I_p-I_q|, factor in double, truncated (gft_ift.cpp:965-973).
+// WB_GRAD:  OIFT_Multi_GradWeight (exp_gradient_weight.cpp:244-254), all in float:
+//           wf = alpha|I_p-I_q| + (1-alpha)(G_p+G_q)*0.5, (int)(wf (1 +- per/100)), (int)wf
+//           when I_p == I_q, clamped at 0.
+// WB_W:     OIFT(W, A, scn, ...), the int W_p + W_q, factor in double (gft_ift.cpp:1142-1152).
+// WB_WXYZ:  OIFT(Wx, Wy, Wz, scn, ...), W_t[p] + W_t[q] with t the axis of the arc
+//           (gft_ift.cpp:1244-1254); Spheric(1.0) only.
+// Every base is symmetric in p and q, so the two-values-per-edge layout holds for all of
+// them.  This file is compiled without FMA contraction.
+enum : int { WB_IMAGE = 0, WB_GRAD = 1, WB_W = 2, WB_WXYZ = 3 };
 struct GradArgs {
     const int32_t *G;
     float alpha, fpf, fmf;
+    const int32_t *W[3];   // WB_W: W[0]; WB_WXYZ: Wx, Wy, Wz
 };
 
-template <int NADJ, typename IT, typename WT, bool GRAD>
+template <int NADJ, typename IT, typename WT, int BM>
 __global__ __launch_bounds__(WX *WY) void k_weights(const IT *__restrict__ img, Dims d, double fp, double fm,
                                                    GradArgs ga, WT *__restrict__ wa, WT *__restrict__ wb,
                                                    int *__restrict__ ext, uint32_t *__restrict__ overflow)
@@ -55,16 +63,22 @@ __global__ __launch_bounds__(WX *WY) void k_weights(const IT *__restrict__ img, 
                     const int Iq = (int)img[q];
                     const int dd = abs(Ip - Iq);
                     int up, dn, eq;
-                    if constexpr (GRAD) {
+                    if constexpr (BM == WB_GRAD) {
                         const float gav = (float)(ga.G[p] + ga.G[q]) * 0.5f;
                         const float wf = ga.alpha * (float)dd + (1.0f - ga.alpha) * gav;
                         eq = max(0, (int)wf);
                         up = max(0, (int)(wf * ga.fpf));
                         dn = max(0, (int)(wf * ga.fmf));
                     } else {
-                        eq = dd;
-                        up = (int)((double)dd * fp);
-                        dn = (int)((double)dd * fm);
+                        int base = dd;
+                        if constexpr (BM == WB_W) base = ga.W[0][p] + ga.W[0][q];
+                        if constexpr (BM == WB_WXYZ) {
+                            const int t = A::dz(i) != 0 ? 2 : (A::dy(i) != 0 ? 1 : 0);
+                            base = ga.W[t][p] + ga.W[t][q];
+                        }
+                        eq = base;
+                        up = (int)((double)base * fp);
+                        dn = (int)((double)base * fm);
                     }
                     // w_obj(p->q): per_pq=+per; w_obj(q->p): roles of p and q swapped
                     a = (uint32_t)(Ip > Iq ? up : (Ip < Iq ? dn : eq));
@@ -176,11 +190,17 @@ void launch_weights(roift_ctx &c, const Dims &d, const void *img, double fp, dou
                   "volume too large for the weight builder's grid");
     const dim3 grid = weights_grid(d);
     if (ga.G)
-        hipLaunchKernelGGL((k_weights<NADJ, IT, WT, true>), grid, dim3(WX, WY), 0, c.stream, (const IT *)img, d, fp,
-                           fm, ga, (WT *)wa, (WT *)wb, ext, ovf);
+        hipLaunchKernelGGL((k_weights<NADJ, IT, WT, WB_GRAD>), grid, dim3(WX, WY), 0, c.stream, (const IT *)img, d,
+                           fp, fm, ga, (WT *)wa, (WT *)wb, ext, ovf);
+    else if (ga.W[1] && NADJ == 7)
+        hipLaunchKernelGGL((k_weights<NADJ, IT, WT, WB_WXYZ>), grid, dim3(WX, WY), 0, c.stream, (const IT *)img, d,
+                           fp, fm, ga, (WT *)wa, (WT *)wb, ext, ovf);
+    else if (ga.W[0])
+        hipLaunchKernelGGL((k_weights<NADJ, IT, WT, WB_W>), grid, dim3(WX, WY), 0, c.stream, (const IT *)img, d,
+                           fp, fm, ga, (WT *)wa, (WT *)wb, ext, ovf);
     else
-        hipLaunchKernelGGL((k_weights<NADJ, IT, WT, false>), grid, dim3(WX, WY), 0, c.stream, (const IT *)img, d, fp,
-                           fm, ga, (WT *)wa, (WT *)wb, ext, ovf);
+        hipLaunchKernelGGL((k_weights<NADJ, IT, WT, WB_IMAGE>), grid, dim3(WX, WY), 0, c.stream, (const IT *)img, d,
+                           fp, fm, ga, (WT *)wa, (WT *)wb, ext, ovf);
     ROIFT_HIP_CHECK(hipGetLastError());
 }
 
@@ -201,12 +221,17 @@ WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_im
     int *part = buf<int>(c, "w.part", 2 * (size_t)nblk);
     // per_pq / 100.0 in double, as the reference's "w *= (1.0 +- per_pq/100.0)"
     const double fp = 1.0 + (double)per / 100.0, fm = 1.0 - (double)per / 100.0;
-    GradArgs ga{nullptr, 0.0f, 0.0f, 0.0f};
+    GradArgs ga{nullptr, 0.0f, 0.0f, 0.0f, {nullptr, nullptr, nullptr}};
     if (gm && gm->G) {   // (1.0f +- per_pq/100.0f) in float, as the GradWeight experiment
         ga.G = gm->G;
         ga.alpha = gm->alpha;
         ga.fpf = 1.0f + per / 100.0f;
         ga.fmf = 1.0f - per / 100.0f;
+    } else if (gm) {
+        ROIFT_REQUIRE(!gm->Wx || nadj == 7, ROIFT_EUNSUPPORTED, "per-axis weights need Spheric(1.0)");
+        ga.W[0] = gm->Wx ? gm->Wx : gm->W;
+        ga.W[1] = gm->Wx ? gm->Wy : nullptr;
+        ga.W[2] = gm->Wx ? gm->Wz : nullptr;
     }
     const bool i16 = image_dtype == ROIFT_I16, w16 = weight_dtype == ROIFT_U16;
     ev_record(c, 8);

@@ -15,6 +15,8 @@ to.  Re-run only if the inputs below change:
     python tests/golden/make_golden.py blur     # only blur/
     python tests/golden/make_golden.py stages   # only stages/
     python tests/golden/make_golden.py cli      # only cli/ (the reference CLI run on files)
+    python tests/golden/make_golden.py oiftw    # only oiftw/ (OIFT(A, ...), OIFT(W, ...), OIFT(Wx, Wy, Wz, ...))
+    python tests/golden/make_golden.py spacing  # only spacing/ (anisotropic voxel spacing)
 """
 import os
 import sys
@@ -140,6 +142,109 @@ def write_variants():
         print(name, img.shape, "obj voxels", int((oift > 0).sum()), int((relax > 0).sum()))
 
 
+def oiftw_cases():
+    """The two-label OIFT(A, ...) and the per-voxel weight forms OIFT(W, A, ...) and
+    OIFT(Wx, Wy, Wz, ...) on small phantoms: (name, form, shape, seed, adj, per, W scale,
+    image offset).  W is a deterministic gradient image times the scale; a scale of 150
+    pushes the arc weights past 16 bits, an offset makes the image partly negative (the
+    W forms compare intensities only)."""
+    yield "a_0", "A", (12, 14, 16), 71, 1.0, 50.0, 1, 0
+    yield "a_1", "A", (10, 12, 14), 72, 1.75, -30.0, 1, 0
+    yield "w_0", "W", (12, 14, 16), 73, 1.0, 50.0, 1, 0
+    yield "w_1", "W", (10, 12, 14), 74, 1.75, -30.0, 1, 0
+    yield "w_2", "W", (12, 12, 12), 75, 1.0, 70.0, 150, -300
+    yield "w_3", "W", (12, 13, 14), 76, 1.0, 0.0, 1, 0
+    yield "wxyz_0", "Wxyz", (12, 14, 16), 77, 1.0, 50.0, 1, 0
+    yield "wxyz_1", "Wxyz", (12, 12, 14), 78, 1.0, -50.0, 40, -100
+
+
+def write_oiftw():
+    out_dir = os.path.join(HERE, "oiftw")
+    os.makedirs(out_dir, exist_ok=True)
+    for name, form, shape, seed, adj, per, scale, off in oiftw_cases():
+        c = P.small_case(shape, seed, per=per, top=255, adj_r=adj)
+        img = np.ascontiguousarray(c.image, np.int32) + off
+        lab0 = c.label_init()
+        rec = dict(form=np.array(form), image=img, seeds=c.seeds.astype(np.int32),
+                   seed_labels=c.seed_labels.astype(np.int32), adj_r=np.float32(adj), per=np.float32(per))
+        W = Wxyz = None
+        if form == "W":
+            W = grad_image(img - off) * scale
+            rec["W"] = W
+        if form == "Wxyz":
+            Wxyz = tuple(grad_image(np.roll(img - off, k, k)) * scale for k in (2, 1, 0))
+            rec.update(Wx=Wxyz[0], Wy=Wxyz[1], Wz=Wxyz[2])
+        rec["oift_label"] = O.ref_oift_w(img, c.seeds, lab0, adj, per, W=W, Wxyz=Wxyz).astype(np.int8)
+        np.savez_compressed(os.path.join(out_dir, f"{name}.npz"), **rec)
+        print("oiftw", name, form, img.shape, "obj voxels", int((rec["oift_label"] > 0).sum()))
+
+
+def spacing_cases():
+    """Anisotropic voxel spacing: (name, spacing, kind, adj).  It reaches the relaxations'
+    Dpq and the blur taps only."""
+    for i, sp in enumerate([(0.8, 1.2, 2.5), (1.0, 0.5, 0.5), (2.0, 2.0, 2.0)]):
+        yield f"relax6_{i}", sp, "relax", 1.0
+    yield "relax26_0", (0.8, 1.2, 2.5), "relax", 1.75
+    yield "perclass_0", (0.7, 0.9, 1.6), "perclass", 1.0
+    yield "rbf_0", (0.8, 1.2, 2.5), "rbf", 1.0
+    yield "rbf26_0", (1.0, 0.5, 0.5), "rbf", 1.75
+    yield "grad_0", (0.8, 1.2, 2.5), "grad", 1.0
+    yield "binary_0", (0.8, 1.2, 2.5), "binary", 1.0
+    yield "binary26_0", (1.0, 0.5, 0.5), "binary", 1.75
+
+
+def write_spacing():
+    out_dir = os.path.join(HERE, "spacing")
+    os.makedirs(out_dir, exist_ok=True)
+    for k, (name, sp, kind, adj) in enumerate(spacing_cases()):
+        c = P.small_case((12, 14, 16), 90 + k, per=50.0, top=255, adj_r=adj)
+        img = np.ascontiguousarray(c.image, np.int32)
+        lab0 = c.label_init()
+        seeds, slab = c.seeds, c.seed_labels
+        pc = None
+        if kind == "perclass":
+            zs, ys, xs = img.shape
+            extra = np.array([3 + 3 * xs + 3 * xs * ys], np.int64)
+            keep = ~np.isin(seeds, extra)
+            seeds = np.concatenate([seeds[keep][:10], extra, seeds[keep][10:]])
+            slab = np.concatenate([slab[keep][:10], [2], slab[keep][10:]]).astype(np.int32)
+            lab0 = np.full(img.size, -1, np.int32)
+            lab0[seeds] = slab
+            lab0 = lab0.reshape(img.shape)
+            pc = np.array([-50.0, 50.0, 50.0], np.float32)
+        oift = O.ref_oift(img, seeds, lab0, adj, 50.0, pc)
+        rec = dict(kind=np.array(kind), spacing=np.array(sp, np.float32), image=img.astype(np.int16),
+                   seeds=seeds.astype(np.int32), seed_labels=slab.astype(np.int32), adj_r=np.float32(adj),
+                   per=np.float32(50.0), oift_label=oift.astype(np.int8), niter=np.int32(8),
+                   per_class=np.zeros(0, np.float32) if pc is None else pc)
+        if kind in ("relax", "perclass"):
+            rec["relax_label"] = O.ref_relax(img, seeds, oift, 8, adj, 50.0, pc, spacing=sp)
+        elif kind == "rbf":
+            sigma = O.ref_estimate_sigma(img)
+            rec["sigma"] = np.float32(sigma)
+            rec["relax_label"] = O.ref_relax_rbf(img, seeds, oift, 8, sigma, adj, 50.0, spacing=sp)
+        elif kind == "grad":
+            G = grad_image(img)
+            rec["G"] = G
+            rec["alpha"] = np.float32(0.5)
+            go = O.ref_oift_gradweight(img, G, seeds, lab0, adj, 0.5, 50.0)
+            rec["oift_label"] = go.astype(np.int8)
+            rec["relax_label"] = O.ref_relax_gradweight(img, G, seeds, go, 8, adj, 0.5, 50.0, spacing=sp)
+        else:
+            lb = (oift > 0).astype(np.int32)
+            rec["oift_label"] = lb.astype(np.int8)
+            rec["relax_label"] = O.ref_relax_binary(img, seeds, lb, 6, adj, 50.0, spacing=sp)
+            rec["niter"] = np.int32(6)
+        rec["relax_label"] = rec["relax_label"].astype(np.int8)
+        blur_in = img * 16 - 1000
+        rec["blur_in"] = blur_in
+        rec["blur1"] = O.ref_gaussian_blur(blur_in, 1, spacing=sp)
+        rec["blur2"] = O.ref_gaussian_blur(blur_in, 2, spacing=sp)
+        rec["taps"] = O.ref_gauss_kernel(sp)
+        np.savez_compressed(os.path.join(out_dir, f"{name}.npz"), **rec)
+        print("spacing", name, sp, kind, "obj voxels", int((rec["relax_label"] > 0).sum()))
+
+
 def blur_cases():
     rng = np.random.default_rng(77)
     yield "ell", P.small_case((12, 14, 16), 11, per=50.0, top=255).image
@@ -218,14 +323,32 @@ def cli_cases():
     yield "clamped", "vol.scn", img, odd, None, ["0.0", "0", "80"], "out.nii.gz", ["6"]
 
 
+def cli_spacing():
+    """Voxel spacing the CLI cases' volume files carry (gft's default elsewhere)."""
+    return {"aniso_nii": (0.7, 0.9, 2.0), "aniso_scn": (1.5, 1.0, 0.6)}
+
+
+def cli_cases_spacing():
+    c = P.small_case((24, 28, 32), 5, per=50.0)
+    img = c.image.astype(np.int32)
+    zs, ys, xs = img.shape
+    xyz = lambda p: (int(p) % xs, (int(p) // xs) % ys, int(p) // (xs * ys))
+    base = [(*xyz(p), 0, int(lb)) for p, lb in zip(c.seeds[:27], c.seed_labels[:27])]
+    bg = [(*xyz(p), 0, 0) for p in c.seeds[27::97][:12]]
+    yield "aniso_nii", "vol.nii.gz", img, base + bg, None, ["0.5", "10", "50"], "out.nii.gz", ["4"]
+    yield "aniso_scn", "vol.scn", img, base + bg, None, ["-0.4", "8", "40"], "out.scn", []
+
+
 def write_cli():
     import shutil
     root = os.path.join(HERE, "cli")
-    for name, vname, vol, seeds, pol, args, oname, tail in cli_cases():
+    import itertools
+    for name, vname, vol, seeds, pol, args, oname, tail in itertools.chain(cli_cases(), cli_cases_spacing()):
         d = os.path.join(root, name)
         shutil.rmtree(d, ignore_errors=True)
         os.makedirs(d)
-        O.ref_scene_write(os.path.join(d, vname), vol)
+        with O._spacing(cli_spacing().get(name), ref=True):
+            O.ref_scene_write(os.path.join(d, vname), vol)
         with open(os.path.join(d, "seeds.txt"), "w") as f:
             f.write(f"{len(seeds)}\n" + "".join(f"{x} {y} {z} {i} {lb}\n" for x, y, z, i, lb in seeds))
         extra = []
@@ -260,6 +383,12 @@ def main():
     if sys.argv[1:] == ["cli"]:
         write_cli()
         return
+    if sys.argv[1:] == ["oiftw"]:
+        write_oiftw()
+        return
+    if sys.argv[1:] == ["spacing"]:
+        write_spacing()
+        return
     for name, c in cases():
         img = np.ascontiguousarray(c["image"], np.int32)
         lab0 = np.full(img.size, -1, np.int32)
@@ -278,6 +407,8 @@ def main():
     write_blur()
     write_stages()
     write_cli()
+    write_oiftw()
+    write_spacing()
 
 
 if __name__ == "__main__":

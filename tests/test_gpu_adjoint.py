@@ -120,3 +120,64 @@ def test_fp64_adjoint_meets_1e4_against_finite_differences(ctx, shape, seed, adj
         step = float(up.ravel()[i]) - float(dn.ravel()[i])
         fd = (L(sc=up) - L(sc=dn)) / step
         assert abs(g.ravel()[i] - fd) <= 1e-4 * abs(fd) + 1e-12
+
+
+@pytest.mark.parametrize("seed", [100, 101])
+def test_config5_headline_128_fp64_and_fp32(ctx, seed):
+    """The differentiable-relaxation workload at its own size: a 128^3 headline phantom, the
+    exact OIFT forest, 50 iterations, checkpoints every 10, BCE against the Gaussian
+    (sigma 2)-blurred true mask, scale 1, alpha 0.5, RBF sigma = the image's global std.
+    FP64 adjoint: equal to the FP64 oracle's reverse-mode gradient (1e-10 relative) and within
+    1e-4 relative of FP64 central differences (h = 1e-4) for sigma, alpha and the three largest
+    scale-map entries.  FP32 adjoint: within 2e-3 relative of the same FP64 reference."""
+    import time
+    import torch
+    import scipy.ndimage as nd
+    c = P.headline_case(128, seed=seed, niter=50)
+    dev = torch.device("cuda")
+    img = torch.from_numpy(c.image.astype(np.int16)).to(dev)
+    seeds = torch.from_numpy(c.seeds.astype(np.int32)).to(dev)
+    lab_t = torch.from_numpy(c.label_init()).to(dev)
+    ctx.oift_fmax(img, seeds, lab_t, 1.0, c.per, want_cost=False)
+    lab = lab_t.cpu().numpy()
+    assert np.array_equal(lab, O.oift_image(c.image, c.seeds, c.label_init(), 1.0, c.per)["label"])
+    target = nd.gaussian_filter(c.truth.astype(np.float32), 2.0).astype(np.float32)
+    scale = np.ones(c.image.shape, np.float32)
+    sigma = float(np.float32(np.sqrt(c.image.astype(np.float64).var())))
+    alpha, niter, h = 0.5, 50, 1e-4
+    print(f"case cfg5 seed={seed} sigma={sigma:.3f}", flush=True)
+    t0 = time.time()
+    r = O.relax_diff(c.image, c.seeds, lab, 1, niter, scale.astype(np.float64), sigma, alpha,
+                     target.astype(np.float64), 1.0, eps=EPS)
+    print(f"case cfg5 seed={seed}: FP64 oracle {time.time() - t0:.1f} s", flush=True)
+    loss, gsc, gs, ga, f = _gpu64(ctx, c, lab, scale, target, niter, sigma, alpha, 1.0)
+    g = gsc.cpu().numpy()
+    assert np.abs(f.cpu().numpy() - r["f"]).max() <= 1e-12
+    assert abs(loss - r["loss"]) <= 1e-12 * abs(r["loss"])
+    assert abs(gs - r["g_sigma"]) <= 1e-10 * abs(r["g_sigma"])
+    assert abs(ga - r["g_alpha"]) <= 1e-10 * abs(r["g_alpha"])
+    assert np.abs(g - r["g_scale"]).max() <= 1e-10 * np.abs(r["g_scale"]).max()
+
+    def L(s=sigma, a=alpha, sc=scale):
+        return _gpu64(ctx, c, lab, sc, target, niter, s, a, 1.0)[0]
+
+    fd_s = (L(s=sigma + h) - L(s=sigma - h)) / (2 * h)
+    fd_a = (L(a=alpha + h) - L(a=alpha - h)) / (2 * h)
+    print(f"case cfg5 seed={seed}: sigma {gs:.6e} vs fd {fd_s:.6e}; alpha {ga:.6e} vs fd {fd_a:.6e}", flush=True)
+    assert abs(gs - fd_s) <= 1e-4 * abs(fd_s)
+    assert abs(ga - fd_a) <= 1e-4 * abs(fd_a)
+    for i in np.argsort(-np.abs(g.ravel()))[:3]:
+        hs = np.float32(1e-2)
+        up, dn = scale.copy(), scale.copy()
+        up.ravel()[i] += hs
+        dn.ravel()[i] -= hs
+        step = float(up.ravel()[i]) - float(dn.ravel()[i])
+        fd = (L(sc=up) - L(sc=dn)) / step
+        assert abs(g.ravel()[i] - fd) <= 1e-4 * abs(fd) + 1e-12
+    # the FP32 form (the one bench.py times) against the same FP64 reference
+    l32, g32, gs32, ga32, f32 = _gpu(ctx, c, lab, scale, target, niter, sigma, alpha, 1.0, 10)
+    assert np.abs(f32.cpu().numpy() - r["f"]).max() <= 1e-5
+    assert abs(l32 - r["loss"]) <= 1e-5 * abs(r["loss"])
+    assert abs(gs32 - r["g_sigma"]) <= 2e-3 * abs(r["g_sigma"])
+    assert abs(ga32 - r["g_alpha"]) <= 2e-3 * abs(r["g_alpha"])
+    assert np.abs(g32.cpu().numpy().astype(np.float64) - r["g_scale"]).max() <= 2e-3 * np.abs(r["g_scale"]).max()

@@ -98,3 +98,17 @@ def test_batch_headline_config_64_volumes(ctx):
     print("total voxels", sum(c.image.size for c in cases), flush=True)
     plain = _plain(ctx, cases, 30)
     _same(_batched(ctx, cases, 30, True), plain)
+
+
+def test_batch_graph_object_label_change_rebuilds(ctx):
+    """Two same-shaped graph calls whose object label changes (1, then 2): the second must
+    not replay launches captured with the first call's label (each equals the eager path)."""
+    cases = P.batch_cases(count=4, seed=16, lo=12, hi=24, niter=4)
+    for lab_obj in (1, 2, 1):
+        for c in cases:
+            c.seed_labels = np.where(c.seed_labels > 0, lab_obj, 0).astype(c.seed_labels.dtype)
+        eager = _batched(ctx, cases, 4, False)
+        graphed = _batched(ctx, cases, 4, True)
+        _same(graphed, eager)
+        for lab, _ in graphed:
+            assert set(np.unique(lab)) <= {0, lab_obj}, f"labels {np.unique(lab)} with object label {lab_obj}"

@@ -1,0 +1,218 @@
+"""GPU parity of the per-voxel weight OIFT forms, of anisotropic voxel spacing and of the
+reference's two legacy C entry points.
+
+Bars: OIFT labels and f_max costs bit-exact (fixtures written by the compiled reference and
+the oracle); relaxation memberships within 1e-5 of the FP64 oracle at the same spacing, labels
+equal to the reference's except where the FP64 top two memberships are within 2e-5; Gaussian
+blur outputs bit-exact.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import roift_amd as R
+from roift_amd import phantom as P
+from test_oracle_forms import load, oracle_oiftw, oracle_spacing_relax
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+OIFTW = sorted(glob.glob(os.path.join(HERE, "golden", "oiftw", "*.npz")))
+SPACING = sorted(glob.glob(os.path.join(HERE, "golden", "spacing", "*.npz")))
+TOL = 1e-5
+
+
+def _dev(a, dtype=None):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def _near_tie(m64, tol=2e-5):
+    s = np.sort(m64, axis=0)
+    return (s[-1] - s[-2]) < tol
+
+
+def _run_oiftw(ctx, c, image_dtype=np.int32):
+    import torch
+    img = _dev(c["image"].astype(image_dtype))
+    seeds = _dev(c["seeds"].astype(np.int32))
+    lab = _dev(c["label0"].astype(np.int32))
+    form = str(c["form"])
+    if form == "A":
+        cost = ctx.oift_fmax(img, seeds, lab, float(c["adj_r"]), float(c["per"]))
+    elif form == "W":
+        cost = ctx.oift_w(img, seeds, lab, float(c["adj_r"]), float(c["per"]), W=_dev(c["W"], torch.int32))
+    else:
+        cost = ctx.oift_w(img, seeds, lab, 1.0, float(c["per"]),
+                          Wxyz=tuple(_dev(c[k], torch.int32) for k in ("Wx", "Wy", "Wz")))
+    return lab.cpu().numpy(), cost.cpu().numpy()
+
+
+@pytest.mark.parametrize("path", OIFTW, ids=lambda p: os.path.basename(p)[:-4])
+def test_oiftw_matches_reference_fixture_and_oracle(ctx, path):
+    c = load(path)
+    print(f"case {c['name']} form={c['form']} shape={c['image'].shape}", flush=True)
+    lab, cost = _run_oiftw(ctx, c)
+    o = oracle_oiftw(c)
+    assert np.array_equal(lab, c["oift_label"].astype(np.int32))
+    assert np.array_equal(cost, o["cost"])
+    if c["image"].min() >= -32768 and c["image"].max() < 32768:
+        lab16, cost16 = _run_oiftw(ctx, c, np.int16)
+        assert np.array_equal(lab16, lab) and np.array_equal(cost16, cost)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_oiftw_random_vs_oracle(ctx, seed):
+    import torch
+    rng = np.random.default_rng(900 + seed)
+    shape = (int(rng.integers(20, 40)), int(rng.integers(20, 40)), int(rng.integers(20, 40)))
+    adj = [1.0, 1.75][seed % 2]
+    per = [50.0, -35.0, 0.0, 90.0][seed]
+    c = P.small_case(shape, 950 + seed, per=per, adj_r=adj, top=4095)
+    I = c.image.astype(np.int32)
+    lab0 = c.label_init()
+    big = 40000 if seed == 3 else 600   # seed 3: arc weights beyond 16 bits
+    W = rng.integers(0, big, shape).astype(np.int32)
+    print(f"case seed={seed} shape={shape} adj={adj} per={per} Wmax={big}", flush=True)
+    lab = _dev(lab0)
+    cost = ctx.oift_w(_dev(I.astype(np.int16)), _dev(c.seeds.astype(np.int32)), lab, adj, per,
+                      W=_dev(W, torch.int32))
+    o = O.oift_w(I, c.seeds, lab0, adj, per, W=W)
+    assert np.array_equal(lab.cpu().numpy(), o["label"]) and np.array_equal(cost.cpu().numpy(), o["cost"])
+    Wxyz = tuple(rng.integers(0, big, shape).astype(np.int32) for _ in range(3))
+    lab = _dev(lab0)
+    cost = ctx.oift_w(_dev(I), _dev(c.seeds.astype(np.int32)), lab, 1.0, per,
+                      Wxyz=tuple(_dev(w, torch.int32) for w in Wxyz))
+    o = O.oift_w(I, c.seeds, lab0, 1.0, per, Wxyz=Wxyz)
+    assert np.array_equal(lab.cpu().numpy(), o["label"]) and np.array_equal(cost.cpu().numpy(), o["cost"])
+
+
+def test_oiftw_rejects_bad_weights(ctx):
+    import torch
+    c = P.small_case((8, 9, 10), 3)
+    img = _dev(c.image.astype(np.int16))
+    seeds = _dev(c.seeds.astype(np.int32))
+    W = np.ones(c.image.shape, np.int32)
+    W[2, 3, 4] = -1
+    with pytest.raises(R.RoiftError) as e:
+        ctx.oift_w(img, seeds, _dev(c.label_init()), 1.0, 50.0, W=_dev(W, torch.int32))
+    assert e.value.code == R.EINVAL
+    ok = _dev(np.ones(c.image.shape, np.int32))
+    with pytest.raises(R.RoiftError):   # W and per-axis weights together
+        ctx.oift_w(img, seeds, _dev(c.label_init()), 1.0, 50.0, W=ok, Wxyz=(ok, ok, ok))
+
+
+@pytest.mark.parametrize("path", SPACING, ids=lambda p: os.path.basename(p)[:-4])
+def test_spacing_matches_reference_fixture(ctx, path):
+    import torch
+    c = load(path)
+    sp = tuple(float(v) for v in c["spacing"])
+    kind, adj, per, niter = str(c["kind"]), float(c["adj_r"]), float(c["per"]), int(c["niter"])
+    print(f"case {c['name']} kind={kind} spacing={sp} adj={adj}", flush=True)
+    img = _dev(c["image"].astype(np.int16))
+    seeds = _dev(c["seeds"].astype(np.int32))
+    lab = _dev(c["oift_label"].astype(np.int32))
+    if kind in ("relax", "perclass"):
+        pc = c["per_class"] if c["per_class"].size else None
+        memb = ctx.relax(img, seeds, lab, niter, adj, per, pc, spacing=sp).cpu().numpy()
+    elif kind == "rbf":
+        memb, _ = ctx.relax_rbf(img, seeds, lab, niter, float(c["sigma"]), adj, spacing=sp)
+        memb = memb.cpu().numpy()
+    elif kind == "grad":
+        memb = ctx.relax_gradweight(img, _dev(c["G"], torch.int32), seeds, lab, niter, float(c["alpha"]), adj, per,
+                                    spacing=sp).cpu().numpy()
+    else:
+        memb = ctx.relax_binary(img, seeds, lab, niter, adj, per, spacing=sp).cpu().numpy()[None]
+    got = lab.cpu().numpy()
+    want = c["relax_label"].astype(np.int32)
+    r64 = oracle_spacing_relax(c, fp64=kind != "binary")
+    if kind == "binary":
+        r32 = oracle_spacing_relax(c)
+        assert np.abs(memb[0] - r32["memb"]).max() <= 1e-6
+        assert np.array_equal(got, want)
+    else:
+        m64 = r64["memb"]
+        assert np.abs(memb[: m64.shape[0]].astype(np.float64) - m64).max() <= TOL
+        diff = got != want
+        assert not diff[~_near_tie(m64)].any()
+    for passes, key in ((1, "blur1"), (2, "blur2")):
+        out = ctx.gaussian_blur(_dev(c["blur_in"].astype(np.int32)), passes=passes, spacing=sp)
+        assert np.array_equal(out.cpu().numpy(), c[key]), key
+
+
+@pytest.mark.parametrize("sp", [(0.8, 1.2, 2.5), (1.0, 0.5, 0.5)])
+def test_spacing_headline_kernel_64(ctx, sp):
+    """The two-class 6-adjacency z-marching kernel with per-axis Dpq on a 64^3 phantom:
+    memberships within 1e-5 of the FP64 oracle at the same spacing; the blur bit-exact."""
+    c = P.headline_case(64, niter=20)
+    I = c.image.astype(np.int32)
+    o = O.oift_image(I, c.seeds, c.label_init(), 1.0, c.per)["label"]
+    lab = _dev(o)
+    memb = ctx.relax(_dev(c.image.astype(np.int16)), _dev(c.seeds.astype(np.int32)), lab, 20, 1.0, c.per,
+                     spacing=sp).cpu().numpy()
+    r64 = O.relax(I, c.seeds, o, 20, 1.0, c.per, fp64=True, spacing=sp)
+    assert np.abs(memb[:2].astype(np.float64) - r64["memb"]).max() <= TOL
+    unit = O.relax(I, c.seeds, o, 20, 1.0, c.per, fp64=True)
+    assert np.abs(unit["memb"] - r64["memb"]).max() > 1e-3   # the spacing matters here
+    out = ctx.gaussian_blur(_dev(c.image.astype(np.int16)), passes=2, spacing=sp)
+    assert np.array_equal(out.cpu().numpy().astype(np.int32), O.gaussian_blur(I, 2, sp))
+
+
+def _legacy_expected(I, h_labels, seeds, pc, gpu_run_rules):
+    n = I.size
+    lab = np.full(n, -1, np.int32)
+    keep = []
+    for p in seeds:
+        if p < 0 or p >= n or lab[p] != -1:
+            continue
+        lb = max(int(h_labels.ravel()[p]), 0)
+        if gpu_run_rules and lb > pc.size - 1:
+            lb = 0
+        lab[p] = lb
+        keep.append(p)
+    o = O.oift_image(I, np.array(keep, np.int64), lab.reshape(I.shape), 1.0, per_class=pc)["label"]
+    return np.where(o < 0, 0, o).astype(np.int32)
+
+
+@pytest.mark.parametrize("entry", ["oift_fmax_gpu_run", "oift_gpu_run"])
+def test_legacy_entry_contract(entry):
+    c = P.small_case((18, 20, 22), 41, per=50.0)
+    I = c.image.astype(np.int32)
+    zs, ys, xs = I.shape
+    n = I.size
+    rng = np.random.default_rng(5)
+    h = rng.integers(-1, 3, n).astype(np.int32)           # non-seed garbage: must be ignored
+    h[c.seeds] = c.seed_labels
+    h[c.seeds[0]] = -4                                      # negative seed label -> 0
+    h[c.seeds[1]] = 7                                       # above max_label
+    seeds = np.concatenate([c.seeds[:5], c.seeds[2:4], c.seeds[5:]]).astype(np.int32)   # repeats
+    if entry == "oift_gpu_run":
+        seeds = np.concatenate([[n + 10, -3], seeds]).astype(np.int32)                    # out of range
+    pc = np.array([-50.0, 50.0], np.float32)
+    want = _legacy_expected(I, h, seeds, pc, entry == "oift_gpu_run")
+    got = h.copy()
+    print("case", entry, I.shape, len(seeds), flush=True)
+    if entry == "oift_gpu_run":
+        rc = R.oift_gpu_run(0, I, got, xs, ys, zs, pc, seeds)
+        assert R.oift_gpu_run(1, I, h.copy(), xs, ys, zs, pc, seeds) == -1   # additive mode
+    else:
+        rc = R.oift_fmax_gpu_run(I, got, xs, ys, zs, pc, seeds)
+    assert rc > 0
+    assert np.array_equal(got, want)
+
+
+def test_legacy_entries_without_valid_seeds():
+    I = np.arange(60, dtype=np.int32).reshape(3, 4, 5)
+    h = np.full(60, 2, np.int32)
+    pc = np.array([-50.0, 50.0], np.float32)
+    got = h.copy()
+    assert R.oift_gpu_run(0, I, got, 5, 4, 3, pc, np.array([60, -1], np.int32)) == 0
+    assert np.array_equal(got, h)                            # left as it is (oift_gpu.cu:207-210)
+    got = h.copy()
+    assert R.oift_fmax_gpu_run(I, got, 5, 4, 3, pc, np.zeros(0, np.int32)) == 0
+    assert not got.any()                                     # every word unpacks label 0

@@ -24,7 +24,7 @@ def declared_symbols():
 
 def test_library_exports_every_declared_symbol():
     names = declared_symbols()
-    assert "roift_oift_fmax_dev" in names and "oift_fmax_gpu_run" in names
+    assert "roift_oift_fmax_dev" in names and "oift_fmax_gpu_run" in names and "oift_gpu_run" in names
     L = R.lib()
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
@@ -45,6 +45,11 @@ GFT_SIGNATURES = {
     "gft::ift::ORelax_1(gft::Scene32::sScene32*, gft::Scene32::sScene32*, gft::Scene32::sScene32*, "
     "gft::Scene32::sScene32*, float, int*, gft::Scene32::sScene32*, int)",
     "gft::Scene32::GaussianBlur(gft::Scene32::sScene32*)",
+    "gft::ift::OIFT(gft::AdjRel3::sAdjRel3*, gft::Scene32::sScene32*, float, int*, gft::Scene32::sScene32*)",
+    "gft::ift::OIFT(gft::Scene32::sScene32*, gft::AdjRel3::sAdjRel3*, gft::Scene32::sScene32*, float, int*, "
+    "gft::Scene32::sScene32*)",
+    "gft::ift::OIFT(gft::Scene32::sScene32*, gft::Scene32::sScene32*, gft::Scene32::sScene32*, "
+    "gft::Scene32::sScene32*, float, int*, gft::Scene32::sScene32*)",
 }
 
 

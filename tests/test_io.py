@@ -16,7 +16,7 @@ from roift_amd import phantom as P
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CLI = os.path.join(HERE, "golden", "cli")
-HAVE_REF = O.ref_lib() is not None
+HAVE_REF = O.ref_available()
 
 
 def _phantom():
@@ -168,4 +168,13 @@ def test_seeds_and_polarity_files(tmp_path):
 
 def test_cli_fixtures_present():
     cases = sorted(glob.glob(os.path.join(CLI, "*", "argv.txt")))
-    assert len(cases) == 4
+    assert len(cases) == 6
+
+
+@pytest.mark.parametrize("name,spacing", [("aniso_nii", (0.7, 0.9, 2.0)), ("aniso_scn", (1.5, 1.0, 0.6))])
+def test_cli_anisotropic_fixtures_carry_their_spacing(name, spacing):
+    """The two anisotropic CLI volumes hold the spacing they were written with (the CLI's
+    blur taps and relaxation Dpq read it from the file)."""
+    vol = [f for f in os.listdir(os.path.join(CLI, name)) if f.startswith("vol.")][0]
+    info = R.volume_info(os.path.join(CLI, name, vol))
+    assert np.allclose((info.dx, info.dy, info.dz), spacing, atol=1e-6)

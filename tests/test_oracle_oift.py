@@ -42,7 +42,7 @@ def test_oracle_matches_reference_golden(case):
     assert np.array_equal(r["label"], case["relax_label"].astype(np.int32))
 
 
-@pytest.mark.skipif(O.ref_lib() is None, reason="oracle/_ref not built (no reference checkout)")
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (no reference checkout)")
 @pytest.mark.parametrize("seed", range(6))
 def test_oracle_matches_compiled_reference(seed):
     rng = np.random.default_rng(100 + seed)

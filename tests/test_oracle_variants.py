@@ -75,7 +75,7 @@ def _grad(img):
     return (g * max(int(I.max()), 1) // max(int(g.max()), 1)).astype(np.int32)
 
 
-@pytest.mark.skipif(O.ref_lib() is None, reason="oracle/_ref not built")
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built")
 @pytest.mark.parametrize("seed,adj,alpha,per", [(0, 1.0, 0.5, 50.0), (1, 1.75, 0.2, -40.0), (2, 1.0, 0.9, 0.0)])
 def test_gradweight_vs_compiled_reference(seed, adj, alpha, per):
     c = P.small_case((9, 11, 13), 100 + seed, per=per, adj_r=adj, top=1023)
@@ -90,7 +90,7 @@ def test_gradweight_vs_compiled_reference(seed, adj, alpha, per):
     assert np.array_equal(rr, rm)
 
 
-@pytest.mark.skipif(O.ref_lib() is None, reason="oracle/_ref not built")
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built")
 @pytest.mark.parametrize("seed,adj,sigma", [(0, 1.0, 0.0), (1, 1.75, 0.0), (2, 1.0, 25.0)])
 def test_rbf_vs_compiled_reference(seed, adj, sigma):
     c = P.small_case((9, 11, 13), 200 + seed, per=50.0, adj_r=adj, top=1023)
@@ -102,7 +102,7 @@ def test_rbf_vs_compiled_reference(seed, adj, sigma):
                           O.relax_model(I, c.seeds, o, 7, "rbf", sigma=s, adj_r=adj)["label"])
 
 
-@pytest.mark.skipif(O.ref_lib() is None, reason="oracle/_ref not built")
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built")
 @pytest.mark.parametrize("form", ["image", "image26", "W", "Wxyz", "noseeds"])
 def test_binary_forms_vs_compiled_reference(form):
     c = P.small_case((9, 11, 13), 300, per=50.0, top=1023)
