@@ -40,7 +40,7 @@ def test_cli_matches_reference_cli(case, tmp_path):
 
 
 def test_cli_errors_return_one(tmp_path):
-    shutil.copy(os.path.join(CASES[0], "vol.scn"), tmp_path)
+    shutil.copy(os.path.join(CLI, "binary_scn", "vol.scn"), tmp_path)
     assert R.cli_main([str(tmp_path / "vol.scn"), str(tmp_path / "missing.txt"), 0.5, 1, 50,
                        str(tmp_path / "out.scn")]) == 1
     assert R.cli_main([str(tmp_path / "missing.scn"), "s.txt", 0.5, 1, 50, str(tmp_path / "out.scn")]) == 1

@@ -189,11 +189,17 @@ def test_legacy_entry_contract(entry):
     h = rng.integers(-1, 3, n).astype(np.int32)           # non-seed garbage: must be ignored
     h[c.seeds] = c.seed_labels
     h[c.seeds[0]] = -4                                      # negative seed label -> 0
-    h[c.seeds[1]] = 7                                       # above max_label
     seeds = np.concatenate([c.seeds[:5], c.seeds[2:4], c.seeds[5:]]).astype(np.int32)   # repeats
-    if entry == "oift_gpu_run":
-        seeds = np.concatenate([[n + 10, -3], seeds]).astype(np.int32)                    # out of range
     pc = np.array([-50.0, 50.0], np.float32)
+    if entry == "oift_gpu_run":
+        h[c.seeds[1]] = 7                                   # above max_label: read as 0 (oift_gpu.cu:202)
+        seeds = np.concatenate([[n + 10, -3], seeds]).astype(np.int32)                    # out of range
+    else:
+        # oift_fmax_gpu_run keeps such a label, whose PerClass polarity is 0 (gft_ift.cpp:1058-1062):
+        # a third polarity beside -50 / +50, outside this library, reported as -1
+        bad = h.copy()
+        bad[c.seeds[1]] = 7
+        assert R.oift_fmax_gpu_run(I, bad, xs, ys, zs, pc, seeds) == -1
     want = _legacy_expected(I, h, seeds, pc, entry == "oift_gpu_run")
     got = h.copy()
     print("case", entry, I.shape, len(seeds), flush=True)
@@ -203,7 +209,7 @@ def test_legacy_entry_contract(entry):
     else:
         rc = R.oift_fmax_gpu_run(I, got, xs, ys, zs, pc, seeds)
     assert rc > 0
-    assert np.array_equal(got, want)
+    assert np.array_equal(got, want.ravel())
 
 
 def test_legacy_entries_without_valid_seeds():
