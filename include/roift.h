@@ -118,6 +118,18 @@ ROIFT_API int roift_set_param(roift_ctx *ctx, int param, int value);
 ROIFT_API int roift_weights_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,
                                 void *d_wa, void *d_wb, int weight_dtype, int32_t *h_maxI, int32_t *h_minI);
 
+/* roift_weights_dev without any device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, kernels and results as roift_weights_dev;
+ * its scratch (per-workgroup extrema partials and the overflow flag) is a static device
+ * array of 8 MiB + 256 B that every GPU the library runs on holds until the process ends,
+ * sized for ceil(xsize/64) * ceil(ysize/4) * ceil(zsize/16) <= 2^20 workgroups; larger
+ * volumes return ROIFT_EUNSUPPORTED before any launch.  All calls on one GPU share that
+ * GPU's array (each clears what it uses first), so calls that overlap on one GPU race. */
+ROIFT_API int roift_weights_noalloc_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image,
+                                        int image_dtype, void *d_wa, void *d_wb, int weight_dtype, int32_t *h_maxI,
+                                        int32_t *h_minI);
+
 /* The same layout for the per-voxel weight forms of the OIFT: base W_p + W_q (d_W, int32 N)
  * or, on Spheric(1.0) only, W_t[p] + W_t[q] with t the arc's axis (d_Wx, d_Wy, d_Wz, int32 N
  * each), times (1 +- per/100) in double by the order of the image values I_p, I_q (unchanged

@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -33,7 +34,7 @@ EXPORTS = (
     "roift_sobel_dev", "roift_bfs_edt_dev", "roift_boundary_seeds_dev", "roift_dilation_conditional_dev",
     "roift_io_last_error", "roift_volume_info_read", "roift_volume_read", "roift_volume_write", "roift_seeds_read",
     "roift_polarity_read", "roift_volume_read_dev", "roift_cli_main", "roift_relax_adjoint64_dev",
-    "roift_weights_w_dev", "roift_oift_w_dev", "roift_oift_w",
+    "roift_weights_w_dev", "roift_oift_w_dev", "roift_oift_w", "roift_weights_noalloc_dev",
 )
 
 
@@ -89,6 +90,7 @@ def lib():
         L.roift_oift_profile.argtypes = [vp, dp, ip]
         L.roift_set_param.argtypes = [vp, ip, ip]
         L.roift_weights_dev.argtypes = [vp, pp, vp, ip, vp, vp, ip, i32p, i32p]
+        L.roift_weights_noalloc_dev.argtypes = [vp, pp, vp, ip, vp, vp, ip, i32p, i32p]
         L.roift_oift_fmax_dev.argtypes = [vp, pp, vp, ip, vp, ip, vp, vp]
         L.roift_oift_edges_dev.argtypes = [vp, pp, vp, vp, ip, i8p, vp, ip, vp, vp]
         L.roift_weights_w_dev.argtypes = [vp, pp, vp, ip, vp, vp, vp, vp, vp, vp, ip]
@@ -243,8 +245,7 @@ class Context:
         # During interpreter shutdown the HIP runtime (and a profiler's library wrapped
         # around it) may already be torn down: a HIP call from here would run after
         # them.  The process exit releases the device memory anyway.
-        import sys
-        if sys.is_finalizing():
+        if sys is None or sys.is_finalizing():
             self._h = None
             return
         try:

@@ -189,7 +189,17 @@ ROIFT_API int roift_create(roift_ctx **out, int device)
     }
     c->owns_stream = true;
     for (auto &e : c->ev)
-        if (hipEventCreate(&e) != hipSuccess) return ROIFT_EHIP;
+        if (hipEventCreate(&e) != hipSuccess) {
+            roift_destroy(c);
+            return ROIFT_EHIP;
+        }
+    // once per device, with the device current: the address of the allocation-free entry's
+    // scratch array on this device (checked; the entry reports a failure here)
+    try {
+        c->noalloc_scratch = noalloc_scratch(device);
+    } catch (const Error &e) {
+        c->noalloc_error = e.msg;
+    }
     *out = c;
     return ROIFT_OK;
 }
@@ -269,6 +279,26 @@ ROIFT_API int roift_weights_dev(roift_ctx *ctx, const roift_problem *pb, const v
         const int nadj = nadj_of(pb);
         ROIFT_REQUIRE(!pb->use_per_class, ROIFT_EINVAL, "the weight builder takes a single polarity (per)");
         const WeightInfo wi = build_weights(*ctx, d, nadj, d_image, image_dtype, pb->per, d_wa, d_wb, weight_dtype);
+        if (h_maxI) *h_maxI = wi.maxI;
+        if (h_minI) *h_minI = wi.minI;
+        return ROIFT_OK;
+    });
+}
+
+ROIFT_API int roift_weights_noalloc_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image,
+                                        int image_dtype, void *d_wa, void *d_wb, int weight_dtype, int32_t *h_maxI,
+                                        int32_t *h_minI)
+{
+    return guarded(ctx, [&]() -> int {
+        const Dims d = dims_of(pb);
+        const int nadj = nadj_of(pb);
+        ROIFT_REQUIRE(!pb->use_per_class, ROIFT_EINVAL, "the weight builder takes a single polarity (per)");
+        ROIFT_REQUIRE(ctx->noalloc_scratch != nullptr, ROIFT_EHIP,
+                      "no device address for the scratch array: " + ctx->noalloc_error);
+        // the same kernels as roift_weights_dev; the extrema partials and the overflow flag live
+        // in the static scratch array instead of context buffers, so nothing is allocated
+        const WeightInfo wi = build_weights(*ctx, d, nadj, d_image, image_dtype, pb->per, d_wa, d_wb, weight_dtype,
+                                            nullptr, ctx->noalloc_scratch);
         if (h_maxI) *h_maxI = wi.maxI;
         if (h_minI) *h_minI = wi.minI;
         return ROIFT_OK;

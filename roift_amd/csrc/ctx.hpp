@@ -32,6 +32,10 @@ struct roift_ctx {
     // persistent kernel, all relaxation iterations; [3] = relaxation launches; [4] = Gaussian blur;
     // [5] = the last CLI stage call (Sobel, distance, boundary seeds or dilation), all its kernels
     double ktimes[6] = {0, 0, 0, 0, 0, 0};
+    // this device's copy of the allocation-free weight builder's static scratch array,
+    // resolved at creation (nullptr if hipGetSymbolAddress failed; that entry then errors)
+    int *noalloc_scratch = nullptr;
+    std::string noalloc_error;
 };
 
 namespace roift {
@@ -97,8 +101,16 @@ struct GradModel {
     const int32_t *Wx = nullptr, *Wy = nullptr, *Wz = nullptr;
 };
 
+// d_scratch (nullable): the allocation-free form's scratch (noalloc_scratch); otherwise the
+// extrema partials come from context buffers.
 WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,
-                         void *d_wa, void *d_wb, int weight_dtype, const GradModel *gm = nullptr);
+                         void *d_wa, void *d_wb, int weight_dtype, const GradModel *gm = nullptr,
+                         int *d_scratch = nullptr);
+// Weight-builder workgroups the allocation-free entry's static scratch holds (8 MiB per device).
+constexpr long long NOALLOC_MAX_BLOCKS = 1 << 20;
+// This device's copy of that scratch array, resolved once per device with hipGetSymbolAddress
+// (call with `device` current).
+int *noalloc_scratch(int device);
 WeightInfo image_extrema(roift_ctx &c, const Dims &d, const void *d_image, int image_dtype);
 // Global standard deviation of the image (estimate_sigma, exp_gaussian_rbf_relax.cpp:194-208).
 float image_sigma(roift_ctx &c, const Dims &d, const void *d_image, int image_dtype);

@@ -10,6 +10,9 @@
 //
 // Bound: HBM.  Algorithmic bytes per voxel = sizeof(image) + 2*E*sizeof(weight)
 // (E = 3 forward edges for 6-adjacency, 13 for 26).
+#include <map>
+#include <mutex>
+
 #include "ctx.hpp"
 
 namespace roift {
@@ -206,19 +209,58 @@ void launch_weights(roift_ctx &c, const Dims &d, const void *img, double fp, dou
 
 }  // namespace
 
+// Scratch of the allocation-free builder (roift_weights_noalloc_dev): [0..1] the folded image
+// extrema, [2] the overflow flag, [3] unused, then two ints (max, min) per workgroup.  Every
+// call on a device shares that device's copy, so calls that overlap on one device race.
+constexpr int NOALLOC_HEAD = 64;   // keeps the per-workgroup slots 256-byte aligned
+static __device__ __align__(256) int scratch[NOALLOC_HEAD + 2 * NOALLOC_MAX_BLOCKS];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, int *> g_scratch_ptr;   // device -> address of that device's copy of `scratch`
+}  // namespace
+
+int *noalloc_scratch(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end()) return it->second;
+    // the host-side name `scratch` is not a device address: resolve this device's copy once
+    // (the caller has made `device` current)
+    int *p = nullptr;
+    ROIFT_HIP_CHECK(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+    ROIFT_REQUIRE(p != nullptr, ROIFT_EHIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[device] = p;
+    return p;
+}
+
 WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,
-                         void *d_wa, void *d_wb, int weight_dtype, const GradModel *gm)
+                         void *d_wa, void *d_wb, int weight_dtype, const GradModel *gm, int *d_scratch)
 {
     ROIFT_REQUIRE(nadj == 7 || nadj == 27, ROIFT_EUNSUPPORTED, "adjacency must be Spheric(1.0) or Spheric(1.75)");
     ROIFT_REQUIRE(image_dtype == ROIFT_I16 || image_dtype == ROIFT_I32, ROIFT_EINVAL, "image dtype must be I16 or I32");
     ROIFT_REQUIRE(weight_dtype == ROIFT_U16 || weight_dtype == ROIFT_U32, ROIFT_EINVAL, "weight dtype must be U16 or U32");
-    int *ext = buf<int>(c, "w.ext", 4);
-    uint32_t *ovf = reinterpret_cast<uint32_t *>(ext + 2);
-    const int init[4] = {INT32_MIN, INT32_MAX, 0, 0};
-    ROIFT_HIP_CHECK(hipMemcpyAsync(ext, init, sizeof(init), hipMemcpyHostToDevice, c.stream));
     const dim3 wg = weights_grid(d);
-    const int nblk = (int)(wg.x * wg.y * wg.z);
-    int *part = buf<int>(c, "w.part", 2 * (size_t)nblk);
+    const long long nblk_ll = (long long)wg.x * wg.y * wg.z;
+    int *ext, *part;
+    if (d_scratch) {
+        // caller-provided scratch (the library's static array): sized for NOALLOC_MAX_BLOCKS
+        // workgroups, cleared by a fill kernel (no memset / host copy nodes)
+        ROIFT_REQUIRE(nblk_ll <= NOALLOC_MAX_BLOCKS, ROIFT_EUNSUPPORTED,
+                      "volume needs " + std::to_string(nblk_ll) + " weight-builder workgroups; the allocation-free "
+                      "entry holds scratch for at most " + std::to_string(NOALLOC_MAX_BLOCKS) +
+                      " (ceil(x/64) * ceil(y/4) * ceil(z/16))");
+        ext = d_scratch;
+        part = d_scratch + NOALLOC_HEAD;
+        fill<int>(c, ext, 4, 0);
+    } else {
+        ext = buf<int>(c, "w.ext", 4);
+        const int init[4] = {INT32_MIN, INT32_MAX, 0, 0};
+        ROIFT_HIP_CHECK(hipMemcpyAsync(ext, init, sizeof(init), hipMemcpyHostToDevice, c.stream));
+        part = buf<int>(c, "w.part", 2 * (size_t)nblk_ll);
+    }
+    uint32_t *ovf = reinterpret_cast<uint32_t *>(ext + 2);
+    const int nblk = (int)nblk_ll;
     // per_pq / 100.0 in double, as the reference's "w *= (1.0 +- per_pq/100.0)"
     const double fp = 1.0 + (double)per / 100.0, fm = 1.0 - (double)per / 100.0;
     GradArgs ga{nullptr, 0.0f, 0.0f, 0.0f, {nullptr, nullptr, nullptr}};
