@@ -124,6 +124,12 @@ struct LabelClasses {
 
 int run_oift(roift_ctx &c, const Dims &d, int nadj, const void *d_wa, const void *d_wb, int weight_dtype,
              const LabelClasses &lc, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int32_t *d_cost);
+// OIFT_Multi's forest from the image: the arc weights, their packed slots and the OIFT state
+// in one fused pass where the weights fit 16 bits (planar weights otherwise); wi receives the
+// image extrema.
+int run_oift_image(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,
+                   const LabelClasses &lc, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int32_t *d_cost,
+                   WeightInfo &wi);
 
 struct RelaxSpec {
     int use_per_class = 0;

@@ -243,8 +243,11 @@ __global__ void k_seed_init(State s, const int32_t *__restrict__ S, int ns, long
 
 }  // namespace
 
-int run_oift(roift_ctx &c, const Dims &d, int nadj, const void *d_wa, const void *d_wb, int weight_dtype,
-             const LabelClasses &lc, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int32_t *d_cost)
+namespace {
+
+// The OIFT's device state, in context buffers reused across calls.
+State alloc_state(roift_ctx &c, const Dims &d, int nadj, const LabelClasses &lc, int n_seeds, int32_t *d_label,
+                  int32_t *d_cost)
 {
     ROIFT_REQUIRE(nadj == 7 || nadj == 27, ROIFT_EUNSUPPORTED, "adjacency must be Spheric(1.0) or Spheric(1.75)");
     ROIFT_REQUIRE(d.n > 0 && d.n < (1ll << 31), ROIFT_EINVAL, "volume must hold 1 .. 2^31-1 voxels");
@@ -288,6 +291,15 @@ int run_oift(roift_ctx &c, const Dims &d, int nadj, const void *d_wa, const void
     s.lcls = lcls;
     s.lcls_n = ncls;
     s.lcls_default = lc.default_cls;
+    return s;
+}
+
+}  // namespace
+
+int run_oift(roift_ctx &c, const Dims &d, int nadj, const void *d_wa, const void *d_wb, int weight_dtype,
+             const LabelClasses &lc, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int32_t *d_cost)
+{
+    State s = alloc_state(c, d, nadj, lc, n_seeds, d_label, d_cost);
     if (weight_dtype == ROIFT_U16) {
         if (nadj == 7) return drive<7, uint16_t>(c, s, d, (const uint16_t *)d_wa, (const uint16_t *)d_wb, d_seeds, n_seeds);
         return drive<27, uint16_t>(c, s, d, (const uint16_t *)d_wa, (const uint16_t *)d_wb, d_seeds, n_seeds);
@@ -295,6 +307,79 @@ int run_oift(roift_ctx &c, const Dims &d, int nadj, const void *d_wa, const void
     ROIFT_REQUIRE(weight_dtype == ROIFT_U32, ROIFT_EINVAL, "weight dtype must be U16 or U32");
     if (nadj == 7) return drive<7, uint32_t>(c, s, d, (const uint32_t *)d_wa, (const uint32_t *)d_wb, d_seeds, n_seeds);
     return drive<27, uint32_t>(c, s, d, (const uint32_t *)d_wa, (const uint32_t *)d_wb, d_seeds, n_seeds);
+}
+
+namespace {
+
+template <int NADJ, typename IT>
+void launch_prep(roift_ctx &c, State &s, const Dims &d, const void *img, double fp, double fm, uint32_t *wpk,
+                 int *part, uint32_t *ovf)
+{
+    const dim3 grid((d.xs + PX - 1) / PX, (d.ys + PY - 1) / PY, (d.zs + PZ - 1) / PZ);
+    hipLaunchKernelGGL((k_prep<NADJ, IT>), grid, dim3(PX, PY), 0, c.stream, (const IT *)img, s, d, fp, fm, wpk, part,
+                       ovf, 2 * d.n + 1024);
+    ROIFT_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+int run_oift_image(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,
+                   const LabelClasses &lc, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int32_t *d_cost,
+                   WeightInfo &wi)
+{
+    ROIFT_REQUIRE(image_dtype == ROIFT_I16 || image_dtype == ROIFT_I32, ROIFT_EINVAL, "image dtype must be I16 or I32");
+    const dim3 grid((d.xs + PX - 1) / PX, (d.ys + PY - 1) / PY, (d.zs + PZ - 1) / PZ);
+    const long long nblk = (long long)grid.x * grid.y * grid.z;
+    // the brick flood reads the planar weights; grids beyond the launch limits take the planar path too
+    const bool fused = c.tune_flood_mode == 0 && grid.y <= 65535 && grid.z <= 65535;
+    if (fused) {
+        State s = alloc_state(c, d, nadj, lc, n_seeds, d_label, d_cost);
+        uint32_t *wpk = buf<uint32_t>(c, "oift.wpk", (size_t)(nadj == 7 ? 8 : 32) * d.n);
+        int *part = buf<int>(c, "oift.prep.part", 2 * (size_t)nblk);
+        int *ext = buf<int>(c, "oift.prep.ext", 4);
+        uint32_t *ovf = reinterpret_cast<uint32_t *>(ext + 2);
+        fill<int>(c, ext, 4, 0);
+        // per_pq / 100.0 in double, as the reference's "w *= (1.0 +- per_pq/100.0)"
+        const double fp = 1.0 + (double)per / 100.0, fm = 1.0 - (double)per / 100.0;
+        const bool i16 = image_dtype == ROIFT_I16;
+        ev_record(c, 8);
+        if (nadj == 7) {
+            if (i16) launch_prep<7, int16_t>(c, s, d, d_image, fp, fm, wpk, part, ovf);
+            else launch_prep<7, int32_t>(c, s, d, d_image, fp, fm, wpk, part, ovf);
+        } else {
+            if (i16) launch_prep<27, int16_t>(c, s, d, d_image, fp, fm, wpk, part, ovf);
+            else launch_prep<27, int32_t>(c, s, d, d_image, fp, fm, wpk, part, ovf);
+        }
+        hipLaunchKernelGGL(k_prep_fold, dim3(1), dim3(1024), 0, c.stream, (const int *)part, (int)nblk, ext);
+        ROIFT_HIP_CHECK(hipGetLastError());
+        ev_record(c, 9);
+        int host[4];
+        ROIFT_HIP_CHECK(hipMemcpyAsync(host, ext, sizeof(host), hipMemcpyDeviceToHost, c.stream));
+        ROIFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+        c.ktimes[0] = ev_ms(c, 8, 9);
+        wi = WeightInfo{host[0], host[1]};
+        ROIFT_REQUIRE(wi.minI >= 0, ROIFT_EINVAL, "image has negative values (shift it to >= 0 first)");
+        if (host[2] == 0) {   // every weight fits 16 bits: the state is ready
+            s.wpk = wpk;
+            if (nadj == 7) return drive<7, uint16_t>(c, s, d, nullptr, nullptr, d_seeds, n_seeds, true);
+            return drive<27, uint16_t>(c, s, d, nullptr, nullptr, d_seeds, n_seeds, true);
+        }
+    }
+    // planar weights (32 bits, or the brick flood), then the OIFT's own initialisation
+    const long long slots = (long long)((nadj - 1) / 2) * d.n;
+    int wdt = ROIFT_U16;
+    void *wa = buf<uint16_t>(c, "pipe.wa", slots), *wb = buf<uint16_t>(c, "pipe.wb", slots);
+    try {
+        wi = build_weights(c, d, nadj, d_image, image_dtype, per, wa, wb, ROIFT_U16);
+    } catch (const Error &e) {
+        if (e.code != ROIFT_EUNSUPPORTED) throw;
+        wdt = ROIFT_U32;
+        wa = buf<uint32_t>(c, "pipe.wa32", slots);
+        wb = buf<uint32_t>(c, "pipe.wb32", slots);
+        wi = build_weights(c, d, nadj, d_image, image_dtype, per, wa, wb, ROIFT_U32);
+    }
+    ROIFT_REQUIRE(wi.minI >= 0, ROIFT_EINVAL, "image has negative values (shift it to >= 0 first)");
+    return run_oift(c, d, nadj, wa, wb, wdt, lc, d_seeds, n_seeds, d_label, d_cost);
 }
 
 }  // namespace roift
