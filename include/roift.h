@@ -82,7 +82,15 @@ ROIFT_API int roift_kernel_timings(const roift_ctx *ctx, double *out, int n);
    entry count (bucket b = ceil(log2(entries)), capped at 15); with n >= 164,
    out[162] = items the flood rounds processed, out[163] = touched-list slots
    the claims scanned; with n >= 166, out[164] / out[165] = flood batches summed
-   over / maximal among the workgroups.  n >= 18.                               */
+   over / maximal among the workgroups; with n >= 167, out[166] = how many OIFT runs of
+   this context in ROIFT_PARAM_OIFT_MODE 1 met a tie and reran level-synchronously.
+   After a run in that mode that did not, out[0..4] are instead its grid rounds,
+   levels, candidate checks, voxels the checks visited and grid-wide flood rounds, and
+   out[5..9] the ms of its phases (frontier, candidates, checks, levels, grid floods) and
+   out[10] the most voxels one workgroup's level settled before any grid-wide flood,
+   out[11] / out[12] the passes of each round's longest check / the steps of its longest
+   level, summed over the rounds.
+   n >= 18.                                                                      */
 ROIFT_API int roift_oift_profile(const roift_ctx *ctx, double *out, int n);
 
 /* Tuning knobs (the autotuner in roift_amd/tune.py sets them):
@@ -95,14 +103,26 @@ ROIFT_API int roift_oift_profile(const roift_ctx *ctx, double *out, int n);
                               bricks relaxed in LDS (6-adjacency only; slower on thin fronts)
    ROIFT_PARAM_RELAX_TILE     two-class 6-adjacency relaxation tile: rows (1..8) | planes << 8
                               | xcd << 16 (1: neighbouring tiles on one XCD); 0 = default
-   ROIFT_PARAM_BLUR_PLANES    Gaussian blur: z planes per workgroup (0 = automatic)         */
+   ROIFT_PARAM_BLUR_PLANES    Gaussian blur: z planes per workgroup (0 = automatic)
+   ROIFT_PARAM_OIFT_MODE      roift_oift_edges_dev: 0 = level-synchronous (default, exact on every
+                              map); 1 = local steps for tie-free maps (pairwise distinct oriented
+                              arc weights, e.g. dense ranks): every level that nothing before it
+                              can reach runs at once, many levels per grid round.  Exact on
+                              tie-free maps; when the run meets a tie it reruns level-synchronously
+                              (same result, the time of both).
+   ROIFT_PARAM_LOCAL_CANDIDATES  OIFT_MODE 1: pending voxels tested per grid round (0 = 256)
+   ROIFT_PARAM_LOCAL_CHECK_STEPS OIFT_MODE 1: passes (512 arcs each) a candidate's test takes before
+                              it gives up (0 = 32; the smallest pending voxel runs regardless)  */
 enum roift_param {
     ROIFT_PARAM_OIFT_BLOCKS = 1,
     ROIFT_PARAM_FLOOD_BUDGET = 2,
     ROIFT_PARAM_RELAX_DEPTH = 3,
     ROIFT_PARAM_FLOOD_MODE = 4,
     ROIFT_PARAM_RELAX_TILE = 5,
-    ROIFT_PARAM_BLUR_PLANES = 6
+    ROIFT_PARAM_BLUR_PLANES = 6,
+    ROIFT_PARAM_OIFT_MODE = 7,
+    ROIFT_PARAM_LOCAL_CANDIDATES = 8,
+    ROIFT_PARAM_LOCAL_CHECK_STEPS = 9
 };
 ROIFT_API int roift_set_param(roift_ctx *ctx, int param, int value);
 

@@ -271,13 +271,31 @@ class Context:
                 "relax_launches": int(out[3]), "blur_ms": out[4], "stage_ms": out[5]}
 
     PARAMS = {"oift_blocks": 1, "flood_budget": 2, "relax_depth": 3, "flood_mode": 4, "relax_tile": 5,
-              "blur_planes": 6}
+              "blur_planes": 6, "oift_mode": 7, "local_candidates": 8, "local_check_steps": 9}
 
     def set_param(self, name, value):
         self._check(self._L.roift_set_param(self._h, self.PARAMS[name], int(value)))
 
     PHASES = ("level_scan", "collect", "key_collect", "key_rank", "flood_seed", "flood_round", "rsafe",
               "claim", "offers")
+
+    def local_profile(self):
+        """The last OIFT in local-step mode (ROIFT_PARAM_OIFT_MODE 1): grid rounds, levels,
+        candidate checks, voxels visited by the checks, grid-wide flood rounds, and the
+        context's count of runs that met a tie and reran level-synchronously, and the ms the
+        lead workgroup spent in each phase (frontier compaction, candidates, checks, levels,
+        grid-wide floods; each up to the exit of the phase's grid barrier)."""
+        out = (C.c_double * 167)()
+        self._check(self._L.roift_oift_profile(self._h, out, 167))
+        keys = ("rounds", "levels", "checks", "check_visits", "grid_flood_rounds")
+        res = {k: int(out[i]) for i, k in enumerate(keys)}
+        for i, k in enumerate(("frontier_ms", "candidates_ms", "checks_ms", "levels_ms", "grid_flood_ms")):
+            res[k] = out[5 + i]
+        res["max_level_voxels"] = int(out[10])
+        res["check_steps"] = int(out[11])      # summed over rounds: the round's longest check
+        res["level_steps"] = int(out[12])      # summed over rounds: the round's longest level
+        res["tie_fallbacks"] = int(out[166])
+        return res
 
     def oift_profile(self, by_size=False):
         """Per-phase device wall-clock ms and counts of the last OIFT (and, with

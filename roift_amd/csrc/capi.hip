@@ -259,7 +259,13 @@ ROIFT_API int roift_set_param(roift_ctx *c, int param, int value)
     case ROIFT_PARAM_RELAX_DEPTH: c->tune_relax_depth = value; return ROIFT_OK;
     case ROIFT_PARAM_FLOOD_MODE: c->tune_flood_mode = value; return ROIFT_OK;
     case ROIFT_PARAM_RELAX_TILE: c->tune_relax_tile = value; return ROIFT_OK;
+    case ROIFT_PARAM_OIFT_MODE:
+        if (value > 1) return ROIFT_EINVAL;
+        c->tune_oift_mode = value;
+        return ROIFT_OK;
     case ROIFT_PARAM_BLUR_PLANES: c->tune_blur_planes = value; return ROIFT_OK;
+    case ROIFT_PARAM_LOCAL_CANDIDATES: c->tune_local_cand = value; return ROIFT_OK;
+    case ROIFT_PARAM_LOCAL_CHECK_STEPS: c->tune_local_steps = value; return ROIFT_OK;
     default: return ROIFT_EINVAL;
     }
 }
@@ -268,6 +274,7 @@ ROIFT_API int roift_oift_profile(const roift_ctx *c, double *out, int n)
 {
     if (!c || !out || n < 18) return ROIFT_EINVAL;
     for (int i = 0; i < n && i < 166; i++) out[i] = c->prof[i];
+    if (n >= 167) out[166] = c->oift_local_fallbacks;
     return ROIFT_OK;
 }
 

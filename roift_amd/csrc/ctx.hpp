@@ -22,6 +22,10 @@ struct roift_ctx {
     int tune_flood_mode = 0;        // OIFT flood: 0 = single voxels, 1 = 8x8x8 bricks (6-adjacency)
     int tune_relax_tile = 0;        // z-marching relaxation tile: rows | planes << 8 | xcd << 16 (0: default)
     int tune_blur_planes = 0;       // Gaussian blur: planes per workgroup (0: automatic)
+    int tune_oift_mode = 0;         // OIFT over explicit weights: 0 level-synchronous, 1 local steps (tie-free maps)
+    int tune_local_cand = 0;        // local steps: candidates per round (0 = 256)
+    int tune_local_steps = 0;       // local steps: steps a check takes before it gives up (0 = 32)
+    int oift_local_fallbacks = 0;   // local-step runs that met a tie and reran level-synchronously
     hipEvent_t ev[16] = {};
     hipGraphExec_t relax_graph = nullptr;   // batched relaxation iterations, replayed while the batch is unchanged
     void *pinned[2] = {nullptr, nullptr};   // pinned host staging for volume reads (double-buffered)

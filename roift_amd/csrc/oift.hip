@@ -240,6 +240,7 @@ __global__ void k_seed_init(State s, const int32_t *__restrict__ S, int ns, long
 }
 
 #include "oift_persistent.inc"
+#include "oift_local.inc"
 
 }  // namespace
 
@@ -299,6 +300,28 @@ State alloc_state(roift_ctx &c, const Dims &d, int nadj, const LabelClasses &lc,
 int run_oift(roift_ctx &c, const Dims &d, int nadj, const void *d_wa, const void *d_wb, int weight_dtype,
              const LabelClasses &lc, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int32_t *d_cost)
 {
+    if (c.tune_oift_mode == 1) {
+        // local steps for tie-free maps; on a tie the labels are restored and the
+        // level-synchronous kernel runs instead
+        ROIFT_REQUIRE(d.n > 0 && d.n < (1ll << 31), ROIFT_EINVAL, "volume must hold 1 .. 2^31-1 voxels");
+        ROIFT_REQUIRE(n_seeds >= 0 && n_seeds <= d.n, ROIFT_EINVAL, "bad seed count");
+        int32_t *lab0 = buf<int32_t>(c, "oift.label_in", d.n);
+        ROIFT_HIP_CHECK(hipMemcpyAsync(lab0, d_label, 4 * d.n, hipMemcpyDeviceToDevice, c.stream));
+        int r;
+        if (weight_dtype == ROIFT_U16)
+            r = nadj == 7 ? drive_local<7, uint16_t>(c, d, (const uint16_t *)d_wa, (const uint16_t *)d_wb, lc, d_seeds,
+                                                     n_seeds, d_label, d_cost)
+                          : drive_local<27, uint16_t>(c, d, (const uint16_t *)d_wa, (const uint16_t *)d_wb, lc,
+                                                      d_seeds, n_seeds, d_label, d_cost);
+        else
+            r = nadj == 7 ? drive_local<7, uint32_t>(c, d, (const uint32_t *)d_wa, (const uint32_t *)d_wb, lc, d_seeds,
+                                                     n_seeds, d_label, d_cost)
+                          : drive_local<27, uint32_t>(c, d, (const uint32_t *)d_wa, (const uint32_t *)d_wb, lc,
+                                                      d_seeds, n_seeds, d_label, d_cost);
+        if (r >= 0) return r;
+        ROIFT_HIP_CHECK(hipMemcpyAsync(d_label, lab0, 4 * d.n, hipMemcpyDeviceToDevice, c.stream));
+        c.oift_local_fallbacks++;
+    }
     State s = alloc_state(c, d, nadj, lc, n_seeds, d_label, d_cost);
     if (weight_dtype == ROIFT_U16) {
         if (nadj == 7) return drive<7, uint16_t>(c, s, d, (const uint16_t *)d_wa, (const uint16_t *)d_wb, d_seeds, n_seeds);
