@@ -389,7 +389,9 @@ def main():
         "oift_phase_count": {k: v[1] for k, v in prof.items() if k in ctx.PHASES},
         "oift_flood_items": prof["flood_items"][0],
         "oift_touched_slots": prof["touched_slots"][0],
-        "oift_flood_batches": {"sum": prof["flood_batches"][0], "max_per_workgroup": prof["flood_batches"][1]},
+        "oift_flood_batches": {"sum": prof["flood_batches"][0], "max_per_workgroup": prof["flood_batches"][1],
+                               "critical_path": prof["flood_critical_batches"][0],
+                               "critical_path_items": prof["flood_critical_batches"][1]},
         "oift_level_steps": stage["oift_steps"],
         "oift_flood_rounds": stage["flood_rounds"],
         "pipeline_roofline": {"bound": "hbm", "algorithmic_bytes_per_voxel": pipe_bytes // n,

@@ -83,7 +83,12 @@ ROIFT_API int roift_kernel_timings(const roift_ctx *ctx, double *out, int n);
    out[162] = items the flood rounds processed, out[163] = touched-list slots
    the claims scanned; with n >= 166, out[164] / out[165] = flood batches summed
    over / maximal among the workgroups; with n >= 167, out[166] = how many OIFT runs of
-   this context in ROIFT_PARAM_OIFT_MODE 1 met a tie and reran level-synchronously.
+   this context in ROIFT_PARAM_OIFT_MODE 1 met a tie and reran level-synchronously; with
+   n >= 168, out[167] = the flood's critical path: the sum over flood rounds of the most
+   batches one workgroup ran in the round; out[168 + b] (n >= 178) = flood batches of
+   ceil(log2(items)) = b, b = 0..9; out[178] (n >= 179) = the items those busiest
+   workgroups ran, summed over the rounds; out[179] / out[180] (n >= 181) = the flood hops
+   run by a single wavefront on a narrow queue and the ms they took, summed over workgroups.
    After a run in that mode that did not, out[0..4] are instead its grid rounds,
    levels, candidate checks, voxels the checks visited and grid-wide flood rounds, and
    out[5..9] the ms of its phases (frontier, candidates, checks, levels, grid floods) and
@@ -99,8 +104,10 @@ ROIFT_API int roift_oift_profile(const roift_ctx *ctx, double *out, int n);
    ROIFT_PARAM_FLOOD_BUDGET   LDS batches a workgroup relaxes per flood round before spilling (0 = 1024)
    ROIFT_PARAM_RELAX_DEPTH    reserved: accepted and ignored (each relaxation iteration is one
                               launch; the kernels are bound by arithmetic, not by HBM)
-   ROIFT_PARAM_FLOOD_MODE     OIFT flood unit: 0 = single voxels (default), 1 = 8x8x8
-                              bricks relaxed in LDS (6-adjacency only; slower on thin fronts)
+   ROIFT_PARAM_FLOOD_MODE     OIFT flood unit: 0 = single voxels (default; a queue of at most
+                              64 items is run by one wavefront without workgroup barriers),
+                              1 = 8x8x8 bricks relaxed in LDS (6-adjacency only; slower on thin
+                              fronts), 2 = single voxels, every batch workgroup-wide (for measurement)
    ROIFT_PARAM_RELAX_TILE     two-class 6-adjacency relaxation tile: rows (1..8) | planes << 8
                               | xcd << 16 (1: neighbouring tiles on one XCD); 0 = default
    ROIFT_PARAM_BLUR_PLANES    Gaussian blur: z planes per workgroup (0 = automatic)

@@ -300,12 +300,15 @@ class Context:
     def oift_profile(self, by_size=False):
         """Per-phase device wall-clock ms and counts of the last OIFT (and, with
         by_size, each phase's ms split by ceil(log2(entries of the step)))."""
-        out = (C.c_double * 166)()
-        self._check(self._L.roift_oift_profile(self._h, out, 166))
+        out = (C.c_double * 181)()
+        self._check(self._L.roift_oift_profile(self._h, out, 181))
         res = {ph: (round(out[i], 3), int(out[9 + i])) for i, ph in enumerate(self.PHASES)}
         res["flood_items"] = (int(out[162]), 0)
         res["touched_slots"] = (int(out[163]), 0)
         res["flood_batches"] = (int(out[164]), int(out[165]))
+        res["flood_critical_batches"] = (int(out[167]), int(out[178]))
+        res["flood_batch_hist"] = tuple(int(out[168 + b]) for b in range(10))
+        res["flood_wave_hops"] = (int(out[179]), round(out[180], 3))
         if by_size:
             res = {ph: [round(out[18 + 16 * i + b], 2) for b in range(16)] for i, ph in enumerate(self.PHASES)}
         return res

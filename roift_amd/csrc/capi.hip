@@ -275,6 +275,8 @@ ROIFT_API int roift_oift_profile(const roift_ctx *c, double *out, int n)
     if (!c || !out || n < 18) return ROIFT_EINVAL;
     for (int i = 0; i < n && i < 166; i++) out[i] = c->prof[i];
     if (n >= 167) out[166] = c->oift_local_fallbacks;
+    if (n >= 168) out[167] = c->prof[166];
+    for (int i = 168; i < n && i < 181; i++) out[i] = c->prof[i - 1];
     return ROIFT_OK;
 }
 

@@ -14,8 +14,11 @@ struct roift_ctx {
     std::map<std::string, std::pair<void *, size_t>> bufs;
     uint32_t *h_ctr = nullptr;      // pinned host mirror of the device counters
     double timings[6] = {0, 0, 0, 0, 0, 0};
-    double prof[166] = {};          // OIFT phase ms [0..8], counts [9..17], ms by log2(entries) [18..161],
-                                    // flood items [162], touched slots [163], batches sum [164] / max per workgroup [165]
+    double prof[180] = {};          // OIFT phase ms [0..8], counts [9..17], ms by log2(entries) [18..161],
+                                    // flood items [162], touched slots [163], batches sum [164] / max per workgroup [165],
+                                    // sum over flood rounds of the busiest workgroup's batches [166],
+                                    // flood batches by ceil(log2(items)) [167..176], items of the rounds' busiest
+                                    // workgroups [177], single-wave flood hops [178] and their ms [179]
     int tune_blocks = 0;            // OIFT persistent grid size (0: sized from the arc count)
     int tune_budget = 0;            // OIFT LDS batches per workgroup per flood round (0: 1024)
     int tune_relax_depth = 0;       // reserved (ROIFT_PARAM_RELAX_DEPTH is accepted and ignored)

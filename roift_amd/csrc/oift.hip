@@ -72,6 +72,7 @@ enum Ctr : int {
     K_BAR_GEN = 33,
     K_HOST = 64,    // words mirrored on the host
     K_CUR = 64,     // line 2: [3] worklist cursors, by round mod 3
+    K_RMAXB = 68,   // [2] u64 by round parity: the round's busiest workgroup, batches << 32 | items (profile)
     K_BAR_ARRIVE = 96,    // line 3
     K_BAR_GROUP = 128,    // lines 4..: [BAR_GROUPS] first-level barrier counters, K_LINE apart
     K_LINE = 32,
@@ -121,7 +122,10 @@ struct State {
 
 constexpr int NPHASE = 9;   // L1 L2 S1 S2 I1 R I2 I3 S3
 // counters in the profile block: flood items and batches processed, touched-list slots claimed
-constexpr int PROF_ITEMS = 240, PROF_TOUCHED = 241, PROF_BATCHES = 242, PROF_BATCHES_MAX = 243;
+constexpr int PROF_ITEMS = 240, PROF_TOUCHED = 241, PROF_BATCHES = 242, PROF_BATCHES_MAX = 243, PROF_CRIT = 244;
+constexpr int PROF_BHIST = 245, NBHIST = 10;   // flood batches by ceil(log2(items)), items 1 .. 512
+constexpr int PROF_CRIT_ITEMS = 255;           // items of the rounds' busiest workgroups
+constexpr int PROF_FAST_HOPS = 176, PROF_FAST_TICKS = 177;   // single-wave flood hops and their wall-clock ticks
 
 __device__ inline int label_class(const State &s, int lb)
 {
@@ -354,7 +358,7 @@ int run_oift_image(roift_ctx &c, const Dims &d, int nadj, const void *d_image, i
     const dim3 grid((d.xs + PX - 1) / PX, (d.ys + PY - 1) / PY, (d.zs + PZ - 1) / PZ);
     const long long nblk = (long long)grid.x * grid.y * grid.z;
     // the brick flood reads the planar weights; grids beyond the launch limits take the planar path too
-    const bool fused = c.tune_flood_mode == 0 && grid.y <= 65535 && grid.z <= 65535;
+    const bool fused = c.tune_flood_mode != 1 && grid.y <= 65535 && grid.z <= 65535;
     if (fused) {
         State s = alloc_state(c, d, nadj, lc, n_seeds, d_label, d_cost);
         uint32_t *wpk = buf<uint32_t>(c, "oift.wpk", (size_t)(nadj == 7 ? 8 : 32) * d.n);

@@ -114,3 +114,27 @@ def test_local_steps_config3_tie_free_96(ctx):
     o = O.oift_edges(ra.cpu().numpy().view(np.uint32), rb.cpu().numpy().view(np.uint32), c.seeds, c.label_init(), 1.75)
     assert prof["tie_fallbacks"] == before
     assert np.array_equal(lab, o["label"]) and np.array_equal(cost, o["cost"])
+
+
+def test_local_steps_config3_tie_free_192(ctx):
+    """The same three-object 26-adjacency volume at 192 x 192 x 128 (4.7M voxels, eight times the
+    case above): about 605k levels in about 10k grid rounds.  The full 384 x 384 x 256 map does not
+    finish within the kernel's 120 s limit (the round count grows about tenfold per doubling of
+    the edge length), so this is the largest tie-free case tested."""
+    import time
+    import torch
+    c = P.multi_object_case(shape=(128, 192, 192), seed=3)
+    img = torch.from_numpy(c.image.astype(np.int16)).cuda()
+    ra, rb = _rank_weights(ctx, img, c.per)
+    del img
+    before = ctx.local_profile()["tie_fallbacks"]
+    print("tie-free 192x192x128 start", flush=True)
+    t0 = time.time()
+    lab, cost, prof, t = _run(ctx, ra, rb, c.seeds, c.label_init(), 1.75)
+    print(f"tie-free 192x192x128: {(time.time() - t0) * 1e3:.1f} ms wall, {prof}", flush=True)
+    wa = ra.cpu().numpy().view(np.uint32)
+    wb = rb.cpu().numpy().view(np.uint32)
+    del ra, rb
+    o = O.oift_edges(wa, wb, c.seeds, c.label_init(), 1.75)
+    assert prof["tie_fallbacks"] == before
+    assert np.array_equal(lab, o["label"]) and np.array_equal(cost, o["cost"])
