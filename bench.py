@@ -130,6 +130,11 @@ def cpu_baseline(size=256, runs=1, second=(128, 3)):
                               "Mvoxels/s": round(nvox2 / t2 / 1e6, 4)}}
 
 
+def reset_params(ctx):
+    for name in ("oift_blocks", "flood_budget", "flood_mode", "relax_tile"):
+        ctx.set_param(name, 0)
+
+
 def batched_bench(ctx, reps=3):
     """64 uneven volumes (dims log-uniform in [64, 192]), OIFT + 30 relaxation iterations each:
     64 plain pipeline calls one after another vs one batched call (HIP-graph relaxation).
@@ -168,18 +173,18 @@ def batched_bench(ctx, reps=3):
             best = dt if best is None else min(best, dt)
         return best
 
-    ctx.set_param("oift_blocks", 0)
-    ctx.set_param("flood_budget", 0)
+    reset_params(ctx)
     t_plain = timed(plain)
     tune = autotune_oift(ctx, "batch64x[64,192]^3", batched, reps=3)
     t_batch = timed(batched)
-    ctx.set_param("oift_blocks", 0)
-    ctx.set_param("flood_budget", 0)
+    graph = ctx.batch_graph_info()
+    reset_params(ctx)
     return {"workload": "64 volumes, dims log-uniform in [64,192], oift_fmax + 30 relax iterations, 6-adj",
             "voxels": nvox, "plain_mvox_s": round(nvox / t_plain / 1e6, 2),
             "batched_mvox_s": round(nvox / t_batch / 1e6, 2), "speedup": round(t_plain / t_batch, 2),
             "plain_ms": round(t_plain * 1e3, 1), "batched_ms": round(t_batch * 1e3, 1),
-            "oift_launch_shape": {"blocks": tune["blocks"], "budget": tune["budget"]}}
+            "tuned": tune["setting"], "graph": {"captures": graph[0], "replays": graph[1], "eager_reruns": graph[2],
+                                               "capture_error": graph[3] or None}}
 
 
 def config3_bench(ctx, reps=2):
@@ -271,6 +276,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-volume batched-vs-plain measurement")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 16 x 128^3 relaxation-adjoint measurement")
     ap.add_argument("--no-config3", action="store_true", help="skip the 384x384x256 26-adjacency measurement")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the headline pipeline's launch parameters (grid, budget, flood mode, relaxation "
+                         "tile) before the timed steps (default: the library's defaults)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -297,6 +305,10 @@ def main():
         lab.copy_(lab0)
         ctx.pipeline(img, seeds, lab, case.niter, 1.0, case.per, cost=cost, memb=memb)
 
+    tuned = None
+    if args.tune:
+        from roift_amd.tune import STAGES_PLAIN, autotune
+        tuned = autotune(ctx, f"headline{args.size}", step, STAGES_PLAIN, reps=3)["setting"]
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -330,7 +342,14 @@ def main():
         out["traffic"] = int(tr * n) if tr is not None else None
         return out
 
+    # one more pass with the OIFT's detailed counters on (flood critical path, batch sizes,
+    # single-wave hops); they cost a few device atomics per flood round, so the timed steps
+    # above run without them
+    ctx.set_param("oift_profile", 1)
+    step()
+    torch.cuda.synchronize()
     prof = ctx.oift_profile()
+    ctx.set_param("oift_profile", 0)
     # the CLI's preprocessing (oiftrelax.cpp:368-375): two Gaussian-blur passes, outside the step
     blurred = torch.empty_like(img)
     blur_ms = []
@@ -381,7 +400,8 @@ def main():
         "dtype": "u16/u64 int (OIFT) + f32 (relax)",
         "data": "synthetic",
         "config": {"workload": case.meta["workload"], "shape": list(img.shape), "image": "int16",
-                   "adjacency": 6, "alpha": 0.5, "relax_iters": case.niter, "seeds": int(seeds.numel())},
+                   "adjacency": 6, "alpha": 0.5, "relax_iters": case.niter, "seeds": int(seeds.numel()),
+                   "launch_params": tuned or "library defaults"},
         "stages_ms": {k: round(float(np.mean(v)), 3) for k, v in stage.items() if isinstance(v, list)},
         "kernel_ms": {"prep_weights_and_state": round(weights_k, 4), "oift_persistent": round(oift_k, 3),
                       "relax_iteration": round(relax_k, 4)},
@@ -391,7 +411,9 @@ def main():
         "oift_touched_slots": prof["touched_slots"][0],
         "oift_flood_batches": {"sum": prof["flood_batches"][0], "max_per_workgroup": prof["flood_batches"][1],
                                "critical_path": prof["flood_critical_batches"][0],
-                               "critical_path_items": prof["flood_critical_batches"][1]},
+                               "critical_path_items": prof["flood_critical_batches"][1],
+                               "single_wave_hops": prof["flood_wave_hops"][0],
+                               "single_wave_hop_ms_summed": prof["flood_wave_hops"][1]},
         "oift_level_steps": stage["oift_steps"],
         "oift_flood_rounds": stage["flood_rounds"],
         "pipeline_roofline": {"bound": "hbm", "algorithmic_bytes_per_voxel": pipe_bytes // n,

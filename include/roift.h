@@ -119,7 +119,11 @@ ROIFT_API int roift_oift_profile(const roift_ctx *ctx, double *out, int n);
                               (same result, the time of both).
    ROIFT_PARAM_LOCAL_CANDIDATES  OIFT_MODE 1: pending voxels tested per grid round (0 = 256)
    ROIFT_PARAM_LOCAL_CHECK_STEPS OIFT_MODE 1: passes (512 arcs each) a candidate's test takes before
-                              it gives up (0 = 32; the smallest pending voxel runs regardless)  */
+                              it gives up (0 = 256; the smallest pending voxel runs unchecked)
+   ROIFT_PARAM_OIFT_PROFILE   1: the persistent OIFT also gathers the flood's critical path, its
+                              batch-size histogram and the single-wave hop times (roift_oift_profile
+                              out[167..180]); they cost a few device atomics per flood round, so the
+                              default (0) leaves them out and reports them as 0                 */
 enum roift_param {
     ROIFT_PARAM_OIFT_BLOCKS = 1,
     ROIFT_PARAM_FLOOD_BUDGET = 2,
@@ -129,7 +133,8 @@ enum roift_param {
     ROIFT_PARAM_BLUR_PLANES = 6,
     ROIFT_PARAM_OIFT_MODE = 7,
     ROIFT_PARAM_LOCAL_CANDIDATES = 8,
-    ROIFT_PARAM_LOCAL_CHECK_STEPS = 9
+    ROIFT_PARAM_LOCAL_CHECK_STEPS = 9,
+    ROIFT_PARAM_OIFT_PROFILE = 10
 };
 ROIFT_API int roift_set_param(roift_ctx *ctx, int param, int value);
 
@@ -256,13 +261,27 @@ ROIFT_API int roift_pipeline_dev(roift_ctx *ctx, const roift_problem *pb, const 
  * volume-local indices, volume after volume, h_nseeds[v] of them.  Every volume's result
  * equals roift_pipeline_dev on that volume alone (bit for bit).  One exact OIFT launch
  * serves the whole batch; the relaxation runs one launch per iteration for all volumes
- * (seed labels {0, L}, intensities < 32767; otherwise volume by volume), replayed as a
- * HIP graph when use_graph != 0.  d_cost nullable.  No reference counterpart: the
- * reference processes one volume per call (oiftrelax.cpp:379-401).                  */
+ * (seed labels {0, L}, intensities < 32767; otherwise volume by volume).  d_cost nullable.
+ * use_graph != 0: the whole pipeline (union layout, weights, the cooperative OIFT kernel,
+ * gather, relaxation and the status copies) is captured as one HIP graph after the first
+ * call with a given key (volume shapes, seed counts, buffer addresses, polarity, iteration
+ * count, tuning parameters) and replayed by the calls that repeat that key, with no host
+ * synchronisation inside; the checks (seeds, negative images, OIFT errors) are read back
+ * after the replay, and weights beyond 16 bits or seed labels other than {0, L} rerun the
+ * call eagerly from the saved input labels.  The first call of a key runs eagerly.  No
+ * reference counterpart: the reference processes one volume per call
+ * (oiftrelax.cpp:379-401).                                                          */
 ROIFT_API int roift_batch_pipeline_dev(roift_ctx *ctx, int nvol, const int32_t *h_dims, float adj_radius, float per,
                                        const void *d_image, int image_dtype, const int32_t *d_seeds,
                                        const int32_t *h_nseeds, int32_t *d_label, int niter, int32_t *d_cost,
                                        int use_graph);
+
+/* State of the batched pipeline's HIP graph in this context: graphs captured, replays, replays
+ * that reran eagerly (weights beyond 16 bits, seed labels other than {0, L}), and, in err
+ * (errlen bytes, nullable), why the last capture failed ("" if none).  No reference
+ * counterpart (the reference has no batched form).                                   */
+ROIFT_API int roift_batch_graph_info(const roift_ctx *ctx, int *captures, int *replays, int *reruns, char *err,
+                                     int errlen);
 
 /* Differentiable two-class relaxation and its adjoint.  No reference counterpart: the
  * learnable weight joins ORelax_1_Multi_GaussRBF's Gaussian (exp_gaussian_rbf_relax.cpp:

@@ -35,6 +35,7 @@ EXPORTS = (
     "roift_io_last_error", "roift_volume_info_read", "roift_volume_read", "roift_volume_write", "roift_seeds_read",
     "roift_polarity_read", "roift_volume_read_dev", "roift_cli_main", "roift_relax_adjoint64_dev",
     "roift_weights_w_dev", "roift_oift_w_dev", "roift_oift_w", "roift_weights_noalloc_dev",
+    "roift_batch_graph_info",
 )
 
 
@@ -109,6 +110,7 @@ def lib():
         L.roift_relax_rbf_dev.argtypes = [vp, pp, vp, ip, fl, vp, ip, vp, ip, vp, fp]
         L.roift_relax_binary.argtypes = [vp, pp, vp, vp, vp, vp, vp, vp, ip, vp, ip, vp]
         L.roift_batch_pipeline_dev.argtypes = [vp, ip, i32p, fl, fl, vp, ip, vp, i32p, vp, ip, vp, ip]
+        L.roift_batch_graph_info.argtypes = [vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip), C.c_char_p, ip]
         L.roift_relax_adjoint_dev.argtypes = [vp, pp, vp, ip, vp, ip, vp, ip, ip, ip, vp, fl, fl, vp, fl, dp, vp, dp,
                                               dp, vp]
         dbl = C.c_double
@@ -271,7 +273,7 @@ class Context:
                 "relax_launches": int(out[3]), "blur_ms": out[4], "stage_ms": out[5]}
 
     PARAMS = {"oift_blocks": 1, "flood_budget": 2, "relax_depth": 3, "flood_mode": 4, "relax_tile": 5,
-              "blur_planes": 6, "oift_mode": 7, "local_candidates": 8, "local_check_steps": 9}
+              "blur_planes": 6, "oift_mode": 7, "local_candidates": 8, "local_check_steps": 9, "oift_profile": 10}
 
     def set_param(self, name, value):
         self._check(self._L.roift_set_param(self._h, self.PARAMS[name], int(value)))
@@ -540,10 +542,23 @@ class Context:
         dev = labels[0].device
         dims = np.array([[t.shape[2], t.shape[1], t.shape[0]] for t in images], np.int32).ravel()
         nseeds = np.array([int(s.numel()) for s in seeds], np.int32)
-        img = torch.cat([t.reshape(-1) for t in images]).contiguous()
-        S = torch.cat([s.to(device=dev, dtype=torch.int32).reshape(-1) for s in seeds]).contiguous()
-        lab = torch.cat([t.reshape(-1) for t in labels]).contiguous()
-        cost = torch.empty_like(lab) if want_cost else None
+        # With use_graph the concatenated inputs live in buffers kept for the next call of the
+        # same shapes: the library's graph is keyed on their addresses and replays while they hold.
+        key = (tuple(dims.tolist()), tuple(nseeds.tolist()), images[0].dtype, bool(want_cost), str(dev))
+        bb = getattr(self, "_batch_bufs", None)
+        if not use_graph or bb is None or bb[0] != key:
+            nvox, nsd = int(sum(t.numel() for t in images)), int(nseeds.sum())
+            bb = (key, torch.empty(nvox, dtype=images[0].dtype, device=dev),
+                  torch.empty(max(nsd, 1), dtype=torch.int32, device=dev), torch.empty(nvox, dtype=torch.int32, device=dev),
+                  torch.empty(nvox, dtype=torch.int32, device=dev) if want_cost else None)
+            if use_graph:
+                self._batch_bufs = bb
+        _, img, S, lab, cost = bb
+        torch.cat([t.reshape(-1) for t in images], out=img)
+        if int(nseeds.sum()) > 0:
+            torch.cat([s.to(device=dev, dtype=torch.int32).reshape(-1) for s in seeds], out=S[:int(nseeds.sum())])
+        torch.cat([t.reshape(-1) for t in labels], out=lab)
+        torch.cuda.current_stream(dev).synchronize()   # the library works on its own stream
         self._check(self._L.roift_batch_pipeline_dev(
             self._h, len(images), dims.ctypes.data_as(C.POINTER(C.c_int32)), 1.0, float(per), img.data_ptr(),
             _dtype_code(images[0]), S.data_ptr(), nseeds.ctypes.data_as(C.POINTER(C.c_int32)), lab.data_ptr(),
@@ -555,6 +570,14 @@ class Context:
             out.append(None if cost is None else cost[o:o + k].view(t.shape))
             o += k
         return out
+
+    def batch_graph_info(self):
+        """The batched pipeline's HIP graph in this context: (captures, replays, eager reruns,
+        the last capture error or "")."""
+        cap, rep, rer = C.c_int(), C.c_int(), C.c_int()
+        err = C.create_string_buffer(512)
+        self._check(self._L.roift_batch_graph_info(self._h, C.byref(cap), C.byref(rep), C.byref(rer), err, 512))
+        return cap.value, rep.value, rer.value, err.value.decode()
 
     def relax_adjoint(self, image, seeds, label, lab_obj, niter, scale, sigma, alpha, target, ckpt_every=10,
                       adj_r=1.0, eps=1e-6, want_memb=False, fp64=False):

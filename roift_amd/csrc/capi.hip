@@ -1,5 +1,6 @@
 // capi.hip -- the extern "C" boundary of libroift.so (declared in include/roift.h).
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <cstdio>
 #include <set>
@@ -213,7 +214,8 @@ ROIFT_API void roift_destroy(roift_ctx *c)
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->h_ctr) (void)hipHostFree(c->h_ctr);
-    if (c->relax_graph) (void)hipGraphExecDestroy(c->relax_graph);
+    if (c->batch_graph) (void)hipGraphExecDestroy(c->batch_graph);
+    if (c->h_bstat) (void)hipHostFree(c->h_bstat);
     for (auto &p : c->pinned)
         if (p) (void)hipHostFree(p);
     for (auto &e : c->io_ev)
@@ -266,6 +268,7 @@ ROIFT_API int roift_set_param(roift_ctx *c, int param, int value)
     case ROIFT_PARAM_BLUR_PLANES: c->tune_blur_planes = value; return ROIFT_OK;
     case ROIFT_PARAM_LOCAL_CANDIDATES: c->tune_local_cand = value; return ROIFT_OK;
     case ROIFT_PARAM_LOCAL_CHECK_STEPS: c->tune_local_steps = value; return ROIFT_OK;
+    case ROIFT_PARAM_OIFT_PROFILE: c->tune_oift_profile = value; return ROIFT_OK;
     default: return ROIFT_EINVAL;
     }
 }
@@ -517,6 +520,22 @@ ROIFT_API int roift_batch_pipeline_dev(roift_ctx *ctx, int nvol, const int32_t *
         return run_batch(*ctx, nvol, h_dims, adj_radius, per, d_image, image_dtype, d_seeds, h_nseeds, d_label, niter,
                          d_cost, use_graph != 0);
     });
+}
+
+ROIFT_API int roift_batch_graph_info(const roift_ctx *c, int *captures, int *replays, int *reruns, char *err,
+                                     int errlen)
+{
+    if (!c) return ROIFT_EINVAL;
+    if (captures) *captures = c->batch_captures;
+    if (replays) *replays = c->batch_replays;
+    if (reruns) *reruns = c->batch_reruns;
+    if (err && errlen > 0) {
+        const std::string &m = c->batch_graph_error;
+        const size_t k = std::min(m.size(), (size_t)errlen - 1);
+        std::memcpy(err, m.data(), k);
+        err[k] = 0;
+    }
+    return ROIFT_OK;
 }
 
 ROIFT_API int roift_relax_adjoint_dev(roift_ctx *ctx, const roift_problem *pb, const void *d_image, int image_dtype,

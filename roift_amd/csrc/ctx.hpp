@@ -27,14 +27,25 @@ struct roift_ctx {
     int tune_blur_planes = 0;       // Gaussian blur: planes per workgroup (0: automatic)
     int tune_oift_mode = 0;         // OIFT over explicit weights: 0 level-synchronous, 1 local steps (tie-free maps)
     int tune_local_cand = 0;        // local steps: candidates per round (0 = 256)
-    int tune_local_steps = 0;       // local steps: steps a check takes before it gives up (0 = 32)
+    int tune_local_steps = 0;       // local steps: steps a check takes before it gives up (0 = 256)
+    int tune_oift_profile = 0;      // 1: the OIFT's flood critical-path / batch / hop counters
     int oift_local_fallbacks = 0;   // local-step runs that met a tie and reran level-synchronously
     hipEvent_t ev[16] = {};
-    hipGraphExec_t relax_graph = nullptr;   // batched relaxation iterations, replayed while the batch is unchanged
+    // the batched pipeline as one HIP graph (scatter, weights, OIFT, gather, relaxation, status
+    // copies), replayed while its key (shapes, pointers, parameters) and every context buffer it
+    // captured are unchanged (buf_gen counts the reallocations)
+    hipGraphExec_t batch_graph = nullptr;
+    std::string batch_graph_key;
+    unsigned long long batch_graph_gen = 0;
+    unsigned long long buf_gen = 0;
+    int *h_bstat = nullptr;         // pinned: the graph's status words (batch info, weight flags)
+    size_t h_bstat_n = 0;
+    int batch_captures = 0, batch_replays = 0, batch_reruns = 0;   // graph builds, replays, eager reruns
+    std::string batch_graph_error;  // why the last capture failed (the call then ran eagerly)
+    std::string batch_graph_fail_key;
     void *pinned[2] = {nullptr, nullptr};   // pinned host staging for volume reads (double-buffered)
     size_t pinned_bytes = 0;
     hipEvent_t io_ev[2] = {};
-    std::string relax_graph_key;
     // device-event times of the last call's main kernels, ms: weight builder, OIFT
     // persistent kernel, all relaxation iterations; [3] = relaxation launches; [4] = Gaussian blur;
     // [5] = the last CLI stage call (Sobel, distance, boundary seeds or dilation), all its kernels
@@ -59,6 +70,7 @@ T *buf(roift_ctx &c, const std::string &name, size_t count)
         ROIFT_HIP_CHECK(hipStreamSynchronize(c.stream));
         ROIFT_HIP_CHECK(hipFree(it->second.first));
         c.bufs.erase(it);
+        c.buf_gen++;   // a captured graph that held the old address is stale
     }
     void *p = nullptr;
     if (hipMalloc(&p, bytes) != hipSuccess) throw Error{ROIFT_ENOMEM, "hipMalloc failed for " + name};
@@ -113,6 +125,11 @@ struct GradModel {
 WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,
                          void *d_wa, void *d_wb, int weight_dtype, const GradModel *gm = nullptr,
                          int *d_scratch = nullptr);
+// The builder's launches alone (no host read-back: graph capture): ext (4 ints) receives the
+// image extrema and the overflow flag, part two ints per workgroup (weights_blocks(d) of them).
+long long weights_blocks(const Dims &d);
+void enqueue_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,
+                     void *d_wa, void *d_wb, int weight_dtype, const GradModel *gm, int *ext, int *part, bool timed);
 // Weight-builder workgroups the allocation-free entry's static scratch holds (8 MiB per device).
 constexpr long long NOALLOC_MAX_BLOCKS = 1 << 20;
 // This device's copy of that scratch array, resolved once per device with hipGetSymbolAddress
@@ -131,6 +148,14 @@ struct LabelClasses {
 
 int run_oift(roift_ctx &c, const Dims &d, int nadj, const void *d_wa, const void *d_wb, int weight_dtype,
              const LabelClasses &lc, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int32_t *d_cost);
+// The same OIFT in three stages for a HIP graph: OIFT_PREPARE allocates every buffer and uploads
+// the class table (before a capture), OIFT_ENQUEUE issues the launches and the counter copy with
+// no host synchronisation (seed errors stop the kernel on the device), OIFT_FINISH (after the
+// graph has run) reports errors and fills the profile.  n_seeds > 0.
+enum : int { OIFT_PREPARE = 0, OIFT_ENQUEUE = 1, OIFT_FINISH = 2 };
+int run_oift_staged(roift_ctx &c, int stage, const Dims &d, int nadj, const void *d_wa, const void *d_wb,
+                    int weight_dtype, const LabelClasses &lc, const int32_t *d_seeds, int n_seeds, int32_t *d_label,
+                    int32_t *d_cost);
 // OIFT_Multi's forest from the image: the arc weights, their packed slots and the OIFT state
 // in one fused pass where the weights fit 16 bits (planar weights otherwise); wi receives the
 // image extrema.
@@ -165,9 +190,6 @@ struct BatchVolume {
     int32_t maxI, minI;
 };
 
-// Two-class 6-adjacency ORelax_1_Multi over a batch in one launch per iteration (labels
-// {0, lab_obj}); d_seeds are flat (batch-global) indices.  use_graph replays the
-// iteration launches as a HIP graph.
 // gft::Scene32::GaussianBlur applied `passes` times (in place when d_in == d_out).
 int run_gaussian_blur(roift_ctx &c, const Dims &d, const void *d_in, int dtype, void *d_out, int passes);
 
@@ -192,9 +214,15 @@ int run_boundary_seeds(roift_ctx &c, const Dims &d, int32_t *d_label, int stride
 int run_dilation_conditional(roift_ctx &c, const Dims &d, float radius, const void *d_image, int dtype,
                              int32_t *d_label, float percentile, int *h_set);
 
+// Two-class 6-adjacency ORelax_1_Multi over a batch in one launch per iteration: d_lab_obj (device)
+// is the object label (classes {0, lab_obj}), d_ext (device) each volume's image max and min;
+// d_seeds are flat indices.  mode: RELAX_BATCH_EAGER uploads the tile and volume tables and runs;
+// RELAX_BATCH_CAPTURE only launches (a HIP graph records them); RELAX_BATCH_TABLES only allocates
+// and uploads the tables (they depend on the shapes only: before a capture and before each replay).
+enum : int { RELAX_BATCH_EAGER = 0, RELAX_BATCH_CAPTURE = 1, RELAX_BATCH_TABLES = 2 };
 int run_relax_batch(roift_ctx &c, const std::vector<BatchVolume> &vols, const void *d_image, int image_dtype,
-                    float per, int lab_obj, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter,
-                    bool use_graph);
+                    float per, const int *d_lab_obj, const int *d_ext, const int32_t *d_seeds, int n_seeds,
+                    int32_t *d_label, int niter, int mode);
 
 // OIFT_Multi + ORelax_1_Multi (6-adjacency, one polarity) over nvol independent volumes
 // stored back to back: h_dims = xs, ys, zs per volume; d_seeds = each volume's seeds as

@@ -252,7 +252,7 @@ namespace {
 
 // The OIFT's device state, in context buffers reused across calls.
 State alloc_state(roift_ctx &c, const Dims &d, int nadj, const LabelClasses &lc, int n_seeds, int32_t *d_label,
-                  int32_t *d_cost)
+                  int32_t *d_cost, bool upload = true)
 {
     ROIFT_REQUIRE(nadj == 7 || nadj == 27, ROIFT_EUNSUPPORTED, "adjacency must be Spheric(1.0) or Spheric(1.75)");
     ROIFT_REQUIRE(d.n > 0 && d.n < (1ll << 31), ROIFT_EINVAL, "volume must hold 1 .. 2^31-1 voxels");
@@ -291,7 +291,7 @@ State alloc_state(roift_ctx &c, const Dims &d, int nadj, const LabelClasses &lc,
     s.cost = d_cost;
     const int ncls = (int)lc.cls.size();
     int8_t *lcls = buf<int8_t>(c, "oift.lcls", ncls > 0 ? ncls : 1);
-    if (ncls > 0)
+    if (ncls > 0 && upload)   // a graph capture uploads the class table before it begins
         ROIFT_HIP_CHECK(hipMemcpyAsync(lcls, lc.cls.data(), ncls, hipMemcpyHostToDevice, c.stream));
     s.lcls = lcls;
     s.lcls_n = ncls;
@@ -300,6 +300,37 @@ State alloc_state(roift_ctx &c, const Dims &d, int nadj, const LabelClasses &lc,
 }
 
 }  // namespace
+
+int run_oift_staged(roift_ctx &c, int stage, const Dims &d, int nadj, const void *d_wa, const void *d_wb,
+                    int weight_dtype, const LabelClasses &lc, const int32_t *d_seeds, int n_seeds, int32_t *d_label,
+                    int32_t *d_cost)
+{
+    ROIFT_REQUIRE(weight_dtype == ROIFT_U16 || weight_dtype == ROIFT_U32, ROIFT_EINVAL, "weight dtype must be U16 or U32");
+    ROIFT_REQUIRE(n_seeds > 0, ROIFT_EINVAL, "the staged OIFT needs seeds");
+    State s = alloc_state(c, d, nadj, lc, n_seeds, d_label, d_cost, stage == OIFT_PREPARE);
+    if (stage == OIFT_PREPARE) {   // every buffer the launches use, allocated before a capture begins
+        (void)buf<uint32_t>(c, "oift.runflag", (size_t)n_seeds);
+        (void)buf<uint32_t>(c, "oift.runtot", (size_t)((n_seeds + 1023) / 1024));
+        if (weight_dtype == ROIFT_U16) (void)buf<uint32_t>(c, "oift.wpk", (size_t)(nadj == 7 ? 8 : 32) * d.n);
+        ROIFT_HIP_CHECK(hipStreamSynchronize(c.stream));
+        return 0;
+    }
+    const bool w16 = weight_dtype == ROIFT_U16;
+    if (stage == OIFT_ENQUEUE) {
+        if (nadj == 7 && w16)
+            drive_enqueue<7, uint16_t>(c, s, d, (const uint16_t *)d_wa, (const uint16_t *)d_wb, d_seeds, n_seeds, false, true);
+        else if (nadj == 7)
+            drive_enqueue<7, uint32_t>(c, s, d, (const uint32_t *)d_wa, (const uint32_t *)d_wb, d_seeds, n_seeds, false, true);
+        else if (w16)
+            drive_enqueue<27, uint16_t>(c, s, d, (const uint16_t *)d_wa, (const uint16_t *)d_wb, d_seeds, n_seeds, false, true);
+        else
+            drive_enqueue<27, uint32_t>(c, s, d, (const uint32_t *)d_wa, (const uint32_t *)d_wb, d_seeds, n_seeds, false, true);
+        return 0;
+    }
+    ROIFT_REQUIRE(stage == OIFT_FINISH, ROIFT_EINVAL, "bad OIFT stage");
+    if (nadj == 7) return w16 ? drive_finish<7, uint16_t>(c, s, false) : drive_finish<7, uint32_t>(c, s, false);
+    return w16 ? drive_finish<27, uint16_t>(c, s, false) : drive_finish<27, uint32_t>(c, s, false);
+}
 
 int run_oift(roift_ctx &c, const Dims &d, int nadj, const void *d_wa, const void *d_wb, int weight_dtype,
              const LabelClasses &lc, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int32_t *d_cost)

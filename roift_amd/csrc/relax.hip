@@ -294,7 +294,8 @@ struct MarchTile {
 struct MarchVol {
     long long off;
     int xs, ys, zs;
-    float wmax;
+    float wmax;   // the volume's float Wmax (set on the device from its image maximum)
+    int lab;      // the object label (set on the device from the seeds)
 };
 
 // DPQ: anisotropic spacing, each neighbour's weight divided by its axis's Dpq (1 = -z, 2 = -y,
@@ -316,6 +317,7 @@ __global__ __launch_bounds__(MX *MY_MAX) void k_relax2_march(const int16_t *__re
         zs = mv.zs;
         base = mv.off;
         a.Wmax = mv.wmax;
+        a.cls_label[1 - a.bg] = mv.lab;
         x = t.x0 + (int)threadIdx.x;
         y = t.y0 + (int)threadIdx.y;
         z0 = t.z0;
@@ -568,9 +570,23 @@ void launch_bin(roift_ctx &c, int bm, const void *img, const BinArgs &a, const f
 }
 
 // ---------------------------------------------------------------- batched two-class relaxation
-__global__ void k_batch_finit(const int32_t *__restrict__ label, long long n, int lab_obj, float *__restrict__ f)
+__global__ void k_batch_finit(const int32_t *__restrict__ label, long long n, const int *__restrict__ lab_obj,
+                              float *__restrict__ f)
 {
-    for (long long p = gtid(); p < n; p += gstride()) f[p] = (label[p] == lab_obj) ? 0.0f : 1.0f;
+    const int lo = *lab_obj;
+    for (long long p = gtid(); p < n; p += gstride()) f[p] = (label[p] == lo) ? 0.0f : 1.0f;
+}
+
+// Each volume's float Wmax = (float)((double)(float)max I * (1.0 + |per| / 100.0)), as
+// gft_ift.cpp:8733-8734 rounds it, from the device extrema (ext[2v] = max of volume v), and the
+// batch's object label.
+__global__ void k_batch_vols(MarchVol *__restrict__ vols, int nvol, const int *__restrict__ ext,
+                             const int *__restrict__ lab_obj, float per)
+{
+    for (int v = (int)gtid(); v < nvol; v += (int)gstride()) {
+        vols[v].wmax = (float)((double)(float)ext[2 * v] * (1.0 + (double)fabsf(per) / 100.0));
+        vols[v].lab = *lab_obj;
+    }
 }
 
 template <typename IT>
@@ -579,10 +595,11 @@ __global__ void k_batch_pack(const IT *__restrict__ img, long long n, int16_t *_
     for (long long p = gtid(); p < n; p += gstride()) out[p] = (int16_t)((int)img[p] & 0x7FFF);
 }
 
-__global__ void k_batch_seedbits(const int32_t *__restrict__ S, int ns, int16_t *__restrict__ packed)
+__global__ void k_batch_seedbits(const int32_t *__restrict__ S, int ns, long long n, int16_t *__restrict__ packed)
 {
     for (long long i = gtid(); i < ns; i += gstride()) {
         const int v = S[i];
+        if (v < 0 || v >= n) continue;   // an invalid seed (reported by the caller's checks)
         packed[v] = (int16_t)((int)packed[v] | 0x8000);
     }
 }
@@ -832,10 +849,11 @@ int run_relax_binary(roift_ctx &c, const Dims &d, int nadj, const void *d_image,
 namespace roift {
 
 int run_relax_batch(roift_ctx &c, const std::vector<BatchVolume> &vols, const void *d_image, int image_dtype,
-                    float per, int lab_obj, const int32_t *d_seeds, int n_seeds, int32_t *d_label, int niter,
-                    bool use_graph)
+                    float per, const int *d_lab_obj, const int *d_ext, const int32_t *d_seeds, int n_seeds,
+                    int32_t *d_label, int niter, int mode)
 {
     ROIFT_REQUIRE(!vols.empty() && niter >= 0, ROIFT_EINVAL, "empty batch or negative iteration count");
+    const bool capture = mode == RELAX_BATCH_CAPTURE;
     hipStream_t st = c.stream;
     long long n = 0;
     std::vector<MarchVol> mv;
@@ -843,10 +861,7 @@ int run_relax_batch(roift_ctx &c, const std::vector<BatchVolume> &vols, const vo
     for (size_t v = 0; v < vols.size(); v++) {
         const BatchVolume &b = vols[v];
         ROIFT_REQUIRE(b.off == n, ROIFT_EINVAL, "batch volumes must be stored back to back");
-        ROIFT_REQUIRE(b.minI >= 0 && b.maxI < 0x7FFF, ROIFT_EUNSUPPORTED, "batched relaxation needs 0 <= I < 32767");
-        // float Wmax = (float)Imax; Wmax *= (1.0 + |per|/100.0)   (gft_ift.cpp:8733-8734), per volume
-        mv.push_back(MarchVol{b.off, b.xs, b.ys, b.zs,
-                              (float)((double)(float)b.maxI * (1.0 + (double)std::fabs(per) / 100.0))});
+        mv.push_back(MarchVol{b.off, b.xs, b.ys, b.zs, 0.0f, 0});   // wmax and lab: k_batch_vols
         for (int z = 0; z < b.zs; z += MZ)
             for (int y = 0; y < b.ys; y += MY)
                 for (int x = 0; x < b.xs; x += MX) tiles.push_back(MarchTile{(int)v, x, y, z});
@@ -855,78 +870,48 @@ int run_relax_batch(roift_ctx &c, const std::vector<BatchVolume> &vols, const vo
     ROIFT_REQUIRE(n < (1ll << 31), ROIFT_EINVAL, "batch larger than 2^31-1 voxels");
     MarchVol *dvol = buf<MarchVol>(c, "batch.vols", mv.size());
     MarchTile *dtile = buf<MarchTile>(c, "batch.tiles", tiles.size());
-    ROIFT_HIP_CHECK(hipMemcpyAsync(dvol, mv.data(), mv.size() * sizeof(MarchVol), hipMemcpyHostToDevice, st));
-    ROIFT_HIP_CHECK(hipMemcpyAsync(dtile, tiles.data(), tiles.size() * sizeof(MarchTile), hipMemcpyHostToDevice, st));
+    if (!capture) {   // a graph's tables are uploaded outside it (they depend on the shapes only)
+        ROIFT_HIP_CHECK(hipMemcpyAsync(dvol, mv.data(), mv.size() * sizeof(MarchVol), hipMemcpyHostToDevice, st));
+        ROIFT_HIP_CHECK(hipMemcpyAsync(dtile, tiles.data(), tiles.size() * sizeof(MarchTile), hipMemcpyHostToDevice, st));
+    }
+    float *f0 = buf<float>(c, "relax.f0", n);
+    float *f1 = buf<float>(c, "relax.f1", n);
+    int16_t *pimg = buf<int16_t>(c, "relax.pimg", n);
+    if (mode == RELAX_BATCH_TABLES) return 0;
     RelaxArgs a{};
     a.d = Dims{0, 0, 0, n};
     a.per = per;
     a.nc = 2;
-    a.bg = 0;   // classes {0, lab_obj} in increasing label order
+    a.bg = 0;   // classes {0, lab_obj} in increasing label order; the object label comes from MarchVol
     a.cls_label[0] = 0;
-    a.cls_label[1] = lab_obj;
     a.tA = (float)(2.0 * (double)per / 100.0);
     a.tB = (float)(-(double)per / 100.0);
-    float *f0 = buf<float>(c, "relax.f0", n);
-    float *f1 = buf<float>(c, "relax.f1", n);
-    int16_t *pimg = buf<int16_t>(c, "relax.pimg", n);
     const unsigned G = grid_for(n, 256, 8192);
-    hipLaunchKernelGGL(k_batch_finit, dim3(G), dim3(256), 0, st, d_label, n, lab_obj, f0);
+    hipLaunchKernelGGL(k_batch_vols, dim3(1), dim3(256), 0, st, dvol, (int)mv.size(), d_ext, d_lab_obj, per);
+    hipLaunchKernelGGL(k_batch_finit, dim3(G), dim3(256), 0, st, d_label, n, d_lab_obj, f0);
     if (image_dtype == ROIFT_I16)
         hipLaunchKernelGGL(k_batch_pack<int16_t>, dim3(G), dim3(256), 0, st, (const int16_t *)d_image, n, pimg);
     else
         hipLaunchKernelGGL(k_batch_pack<int32_t>, dim3(G), dim3(256), 0, st, (const int32_t *)d_image, n, pimg);
     if (n_seeds > 0)
-        hipLaunchKernelGGL(k_batch_seedbits, dim3(grid_for(n_seeds)), dim3(256), 0, st, d_seeds, n_seeds, pimg);
+        hipLaunchKernelGGL(k_batch_seedbits, dim3(grid_for(n_seeds)), dim3(256), 0, st, d_seeds, n_seeds, n, pimg);
     ROIFT_HIP_CHECK(hipGetLastError());
     const unsigned nt = (unsigned)tiles.size();
-    auto launch_all = [&]() {
-        float *fin = f0, *fout = f1;
-        for (int it = 0; it < niter; it++) {
-            int32_t *lo = (it == niter - 1) ? d_label : nullptr;
-            hipLaunchKernelGGL((k_relax2_march<true, false>), dim3(nt), dim3(MX, MY), 0, st, pimg, a, dtile, dvol,
-                               fin, fout, lo);
-            std::swap(fin, fout);
-        }
-    };
-    ev_record(c, 12);
-    bool graphed = false;
-    if (use_graph && niter > 0 && st != nullptr) {
-        // the iteration launches as one HIP graph, re-instantiated when anything a captured
-        // launch holds by value changes: the buffers, the grid, the iteration count and every
-        // byte of the kernel's RelaxArgs (object label, polarity, sizes; `a` is
-        // value-initialised, so its padding is zero and the bytes are deterministic)
-        char head[160];
-        std::snprintf(head, sizeof(head), "%p %p %p %p %p %p %u %d|", (void *)pimg, (void *)dtile, (void *)dvol,
-                      (void *)f0, (void *)f1, (void *)d_label, nt, niter);
-        std::string key(head);
-        const unsigned char *ab = reinterpret_cast<const unsigned char *>(&a);
-        static const char hx[] = "0123456789abcdef";
-        for (size_t i = 0; i < sizeof(a); i++) {
-            key.push_back(hx[ab[i] >> 4]);
-            key.push_back(hx[ab[i] & 15]);
-        }
-        if (!c.relax_graph || c.relax_graph_key != key) {
-            if (c.relax_graph) {
-                ROIFT_HIP_CHECK(hipGraphExecDestroy(c.relax_graph));
-                c.relax_graph = nullptr;
-            }
-            hipGraph_t g = nullptr;
-            ROIFT_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-            launch_all();
-            ROIFT_HIP_CHECK(hipStreamEndCapture(st, &g));
-            ROIFT_HIP_CHECK(hipGraphInstantiate(&c.relax_graph, g, nullptr, nullptr, 0));
-            ROIFT_HIP_CHECK(hipGraphDestroy(g));
-            c.relax_graph_key = key;
-        }
-        ROIFT_HIP_CHECK(hipGraphLaunch(c.relax_graph, st));
-        graphed = true;
+    if (!capture) ev_record(c, 12);
+    float *fin = f0, *fout = f1;
+    for (int it = 0; it < niter; it++) {
+        int32_t *lo = (it == niter - 1) ? d_label : nullptr;
+        hipLaunchKernelGGL((k_relax2_march<true, false>), dim3(nt), dim3(MX, MY), 0, st, pimg, a, dtile, dvol, fin,
+                           fout, lo);
+        std::swap(fin, fout);
     }
-    if (!graphed) launch_all();
     ROIFT_HIP_CHECK(hipGetLastError());
-    ev_record(c, 13);
-    ROIFT_HIP_CHECK(hipStreamSynchronize(st));
-    c.ktimes[2] = ev_ms(c, 12, 13);
-    c.ktimes[3] = niter;
+    if (!capture) {
+        ev_record(c, 13);
+        ROIFT_HIP_CHECK(hipStreamSynchronize(st));
+        c.ktimes[2] = ev_ms(c, 12, 13);
+        c.ktimes[3] = niter;
+    }
     return 0;
 }
 

@@ -234,33 +234,23 @@ int *noalloc_scratch(int device)
     return p;
 }
 
-WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,
-                         void *d_wa, void *d_wb, int weight_dtype, const GradModel *gm, int *d_scratch)
+long long weights_blocks(const Dims &d)
+{
+    const dim3 wg = weights_grid(d);
+    return (long long)wg.x * wg.y * wg.z;
+}
+
+void enqueue_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,
+                     void *d_wa, void *d_wb, int weight_dtype, const GradModel *gm, int *ext, int *part, bool timed)
 {
     ROIFT_REQUIRE(nadj == 7 || nadj == 27, ROIFT_EUNSUPPORTED, "adjacency must be Spheric(1.0) or Spheric(1.75)");
     ROIFT_REQUIRE(image_dtype == ROIFT_I16 || image_dtype == ROIFT_I32, ROIFT_EINVAL, "image dtype must be I16 or I32");
     ROIFT_REQUIRE(weight_dtype == ROIFT_U16 || weight_dtype == ROIFT_U32, ROIFT_EINVAL, "weight dtype must be U16 or U32");
-    const dim3 wg = weights_grid(d);
-    const long long nblk_ll = (long long)wg.x * wg.y * wg.z;
-    int *ext, *part;
-    if (d_scratch) {
-        // caller-provided scratch (the library's static array): sized for NOALLOC_MAX_BLOCKS
-        // workgroups, cleared by a fill kernel (no memset / host copy nodes)
-        ROIFT_REQUIRE(nblk_ll <= NOALLOC_MAX_BLOCKS, ROIFT_EUNSUPPORTED,
-                      "volume needs " + std::to_string(nblk_ll) + " weight-builder workgroups; the allocation-free "
-                      "entry holds scratch for at most " + std::to_string(NOALLOC_MAX_BLOCKS) +
-                      " (ceil(x/64) * ceil(y/4) * ceil(z/16))");
-        ext = d_scratch;
-        part = d_scratch + NOALLOC_HEAD;
-        fill<int>(c, ext, 4, 0);
-    } else {
-        ext = buf<int>(c, "w.ext", 4);
-        const int init[4] = {INT32_MIN, INT32_MAX, 0, 0};
-        ROIFT_HIP_CHECK(hipMemcpyAsync(ext, init, sizeof(init), hipMemcpyHostToDevice, c.stream));
-        part = buf<int>(c, "w.part", 2 * (size_t)nblk_ll);
-    }
+    // ext[0..1] are written by k_fold_maxmin; the overflow flag ext[2] starts at 0 (a fill
+    // kernel, so that a captured graph holds no memset or host-copy node)
+    fill<int>(c, ext, 4, 0);
     uint32_t *ovf = reinterpret_cast<uint32_t *>(ext + 2);
-    const int nblk = (int)nblk_ll;
+    const int nblk = (int)weights_blocks(d);
     // per_pq / 100.0 in double, as the reference's "w *= (1.0 +- per_pq/100.0)"
     const double fp = 1.0 + (double)per / 100.0, fm = 1.0 - (double)per / 100.0;
     GradArgs ga{nullptr, 0.0f, 0.0f, 0.0f, {nullptr, nullptr, nullptr}};
@@ -276,7 +266,7 @@ WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_im
         ga.W[2] = gm->Wx ? gm->Wz : nullptr;
     }
     const bool i16 = image_dtype == ROIFT_I16, w16 = weight_dtype == ROIFT_U16;
-    ev_record(c, 8);
+    if (timed) ev_record(c, 8);
     if (nadj == 7) {
         if (i16 && w16) launch_weights<7, int16_t, uint16_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, part, ovf);
         else if (i16) launch_weights<7, int16_t, uint32_t>(c, d, d_image, fp, fm, ga, d_wa, d_wb, part, ovf);
@@ -290,7 +280,28 @@ WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_im
     }
     hipLaunchKernelGGL(k_fold_maxmin, dim3(1), dim3(1024), 0, c.stream, (const int *)part, nblk, ext);
     ROIFT_HIP_CHECK(hipGetLastError());
-    ev_record(c, 9);
+    if (timed) ev_record(c, 9);
+}
+
+WeightInfo build_weights(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int image_dtype, float per,
+                         void *d_wa, void *d_wb, int weight_dtype, const GradModel *gm, int *d_scratch)
+{
+    const long long nblk_ll = weights_blocks(d);
+    int *ext, *part;
+    if (d_scratch) {
+        // caller-provided scratch (the library's static array): sized for NOALLOC_MAX_BLOCKS
+        // workgroups, cleared by a fill kernel (no memset / host copy nodes)
+        ROIFT_REQUIRE(nblk_ll <= NOALLOC_MAX_BLOCKS, ROIFT_EUNSUPPORTED,
+                      "volume needs " + std::to_string(nblk_ll) + " weight-builder workgroups; the allocation-free "
+                      "entry holds scratch for at most " + std::to_string(NOALLOC_MAX_BLOCKS) +
+                      " (ceil(x/64) * ceil(y/4) * ceil(z/16))");
+        ext = d_scratch;
+        part = d_scratch + NOALLOC_HEAD;
+    } else {
+        ext = buf<int>(c, "w.ext", 4);
+        part = buf<int>(c, "w.part", 2 * (size_t)nblk_ll);
+    }
+    enqueue_weights(c, d, nadj, d_image, image_dtype, per, d_wa, d_wb, weight_dtype, gm, ext, part, true);
     int host[4];
     ROIFT_HIP_CHECK(hipMemcpyAsync(host, ext, sizeof(host), hipMemcpyDeviceToHost, c.stream));
     ROIFT_HIP_CHECK(hipStreamSynchronize(c.stream));
