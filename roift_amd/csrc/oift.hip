@@ -73,6 +73,7 @@ enum Ctr : int {
     K_HOST = 64,    // words mirrored on the host
     K_CUR = 64,     // line 2: [3] worklist cursors, by round mod 3
     K_RMAXB = 68,   // [2] u64 by round parity: the round's busiest workgroup, batches << 32 | items (profile)
+    K_CLAIMED = 72, // [2] by iteration parity: the step was claimed inside the flood's last barrier
     K_BAR_ARRIVE = 96,    // line 3
     K_BAR_GROUP = 128,    // lines 4..: [BAR_GROUPS] first-level barrier counters, K_LINE apart
     K_LINE = 32,
