@@ -40,13 +40,15 @@ ALG = dict(weights=2 + 3 * 2 * 2, oift=12 + 4 + 4 + 4, relax_iter=2 + 4 + 4, lab
            sobel=2 + 4)   # Sobel: read the int16 image, write the int32 gradient
 # HBM bytes per voxel per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
 # workload, scaled by the same counters on a 1 GiB copy (profiles/pmc_traffic_headline256.md).
-TRAFFIC = {"k_oift_persistent": 1489.83, "k_relax2_march": 10.51, "k_weights": 16.13, "k_blur": 4.71,
-           "k_sobel": 7.34}
+TRAFFIC = {"k_oift_persistent": 1503.80, "k_relax2_march": 10.51, "k_weights": 16.13, "k_prep": 80.06,
+           "k_blur": 4.71, "k_sobel": 7.34}
 # What limits each kernel, from its profiles (rocprofv3 counters and A/B timings):
 LIMITED_BY = {
-    "k_oift_persistent": "latency: ~2,100 grid barriers and the flood's dependent atomic round trips; "
-                         "HBM traffic is a by-product of the L2 write-back/invalidate at each barrier",
-    "k_relax2_march": "VALU: ~105 instructions per voxel (the eighth power, the division, six neighbours)",
+    "k_oift_persistent": "latency: ~2,100 grid barriers (2.4 us of atomics plus 1.6-3.5 us of agent-scope "
+                         "L2 write-back/invalidate each) and the flood's dependent chains (1.7 us per "
+                         "single-wave hop); HBM traffic is a by-product of the barriers' write-back/invalidate",
+    "k_relax2_march": "VALU: 94% busy, ~122 lane-instructions per voxel (the eighth power, the division, "
+                      "six neighbours)",
     "k_weights": "HBM",
     "k_prep": "HBM (stores): 69 of its 75 bytes per voxel are writes",
     "k_blur": "VALU: the reference's FP32 tap order, ~50 instructions per output voxel",
