@@ -425,6 +425,100 @@ __global__ __launch_bounds__(MX *MY_MAX) void k_relax2_march(const int16_t *__re
 }
 
 // int16 image with the seed flag in bit 15, for the z-marching form
+// 26-adjacency, two classes, any weight model: each thread owns one (x, y) column of a 64 x my
+// tile and marches a chunk of z planes, keeping the 3 x 3 neighbourhood of planes z-1, z and z+1
+// in registers, so a voxel costs 9 image and 9 membership loads (plane z+1) instead of 26 + 26.
+// A neighbour outside the volume reads the voxel's own column and its weight is zeroed: a +0
+// weight and a +0 product leave both sums bit-identical to skipping it (weights and memberships
+// are >= 0), so the result equals k_relax2's, summed in the reference's Spheric order (the
+// raster order of the 3 x 3 x 3 block without its centre).
+template <typename IT, int WM>
+__global__ __launch_bounds__(MX *MY_MAX) void k_relax2_march26(const IT *__restrict__ img, RelaxArgs a,
+                                                            const float *__restrict__ fin, float *__restrict__ fout,
+                                                            const uint8_t *__restrict__ seedcls,
+                                                            int32_t *__restrict__ label_out)
+{
+    const int xs = a.d.xs, ys = a.d.ys, zs = a.d.zs;
+    const unsigned ntx = (unsigned)(xs + MX - 1) / MX, nty = (unsigned)(ys + a.my - 1) / a.my;
+    const unsigned ntiles = ntx * nty * ((unsigned)(zs + a.mz - 1) / a.mz);
+    unsigned tile = blockIdx.x;
+    if (a.xcd) tile = (blockIdx.x % 8u) * ((ntiles + 7u) / 8u) + blockIdx.x / 8u;
+    if (tile >= ntiles) return;   // no block-level synchronisation below
+    const int x = (int)(tile % ntx) * MX + (int)threadIdx.x;
+    const int y = (int)((tile / ntx) % nty) * a.my + (int)threadIdx.y;
+    const int z0 = (int)(tile / (ntx * nty)) * a.mz;
+    const int z1 = min(zs, z0 + a.mz);
+    if (x >= xs || y >= ys || z0 >= zs) return;   // no block-level synchronisation below
+    const uint32_t uxs = (uint32_t)xs, xy = (uint32_t)xs * (uint32_t)ys;
+    // the 3 x 3 columns around (x, y): c = (dy + 1) * 3 + (dx + 1); absent ones read column 4
+    int32_t off[9];
+    uint32_t msk[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+        const int dx = c % 3 - 1, dy = c / 3 - 1;
+        const bool ok = (unsigned)(x + dx) < (unsigned)xs && (unsigned)(y + dy) < (unsigned)ys;
+        off[c] = ok ? dx + dy * xs : 0;
+        msk[c] = ok ? ~0u : 0u;
+    }
+    int I[3][9] = {};
+    float F[3][9] = {};
+    const uint32_t pc = (uint32_t)x + (uint32_t)y * uxs;   // the column's voxel in plane 0
+    auto load_plane = [&](int slot, int z) {
+        const uint32_t b = pc + (uint32_t)z * xy;
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+            I[slot][c] = (int)img[(uint32_t)((int)b + off[c])];
+            F[slot][c] = fin[(uint32_t)((int)b + off[c])];
+        }
+    };
+    if (z0 > 0) load_plane(0, z0 - 1);
+    load_plane(1, z0);
+    const int lab_bg = a.cls_label[a.bg], lab_ob = a.cls_label[1 - a.bg];
+    for (int z = z0; z < z1; z++) {
+        const bool hm = z > 0, hp = z + 1 < zs;
+        if (hp) load_plane(2, z + 1);
+        const uint32_t p = pc + (uint32_t)z * xy;
+        const uint8_t sc = seedcls[p];
+        float v;
+        if (sc != NOSEED) {
+            v = (sc == a.bg) ? 1.0f : 0.0f;
+        } else {
+            const int Ip = I[1][4];
+            float sw = 0.0f, acc = 0.0f;
+            int k = 1;
+#pragma unroll
+            for (int pl = 0; pl < 3; pl++) {
+                const uint32_t pm = (pl == 0 ? hm : (pl == 2 ? hp : true)) ? ~0u : 0u;
+#pragma unroll
+                for (int c = 0; c < 9; c++) {
+                    if (pl == 1 && c == 4) continue;
+                    const uint32_t m = msk[c] & pm;
+                    const float fq = F[pl][c];
+                    const long long q = m ? (long long)((int)p + off[c] + (pl - 1) * (int)xy) : (long long)p;
+                    // soft polarity from the neighbour's memberships (gft_ift.cpp:8800-8802, 9077-9079)
+                    const float s = a.use_pc ? (a.pcm[a.bg] * fq + a.pcm[1 - a.bg] * (1.0f - fq))
+                                             : a.per * (2.0f * fq - 1.0f);
+                    float w = model_weight<27, WM>(a, k, Ip, I[pl][c], p, q, s);
+                    w = __uint_as_float(__float_as_uint(w) & m);
+                    sw += w;
+                    acc += w * __uint_as_float(__float_as_uint(fq) & m);
+                    k++;
+                }
+            }
+            v = (sw > 0.0f) ? acc / sw : F[1][4];
+        }
+        fout[p] = v;
+        if (label_out) label_out[p] = ((1.0f - v) > v) ? lab_ob : lab_bg;
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+            I[0][c] = I[1][c];
+            F[0][c] = F[1][c];
+            I[1][c] = I[2][c];
+            F[1][c] = F[2][c];
+        }
+    }
+}
+
 template <typename IT>
 __global__ void k_pack_image(const IT *__restrict__ img, const uint8_t *__restrict__ seedcls, long long n,
                              int16_t *__restrict__ out)
@@ -454,6 +548,23 @@ void launch_iter(roift_ctx &c, const void *img, const RelaxArgs &a, int model, c
     if (model == RELAX_GRAD) launch_iter_m<NADJ, IT, RELAX_GRAD>(c, img, a, fin, fout, seedcls, label_out);
     else if (model == RELAX_RBF) launch_iter_m<NADJ, IT, RELAX_RBF>(c, img, a, fin, fout, seedcls, label_out);
     else launch_iter_m<NADJ, IT, RELAX_INTENSITY>(c, img, a, fin, fout, seedcls, label_out);
+}
+
+template <typename IT>
+void launch_march26(roift_ctx &c, const void *img, const RelaxArgs &a, int model, dim3 grid, const float *fin,
+                    float *fout, const uint8_t *seedcls, int32_t *label_out)
+{
+    const dim3 blk(MX, a.my);
+    if (model == RELAX_GRAD)
+        hipLaunchKernelGGL((k_relax2_march26<IT, RELAX_GRAD>), grid, blk, 0, c.stream, (const IT *)img, a, fin, fout,
+                           seedcls, label_out);
+    else if (model == RELAX_RBF)
+        hipLaunchKernelGGL((k_relax2_march26<IT, RELAX_RBF>), grid, blk, 0, c.stream, (const IT *)img, a, fin, fout,
+                           seedcls, label_out);
+    else
+        hipLaunchKernelGGL((k_relax2_march26<IT, RELAX_INTENSITY>), grid, blk, 0, c.stream, (const IT *)img, a, fin,
+                           fout, seedcls, label_out);
+    ROIFT_HIP_CHECK(hipGetLastError());
 }
 
 // ---------------------------------------------------------------- ORelax_1 (binary)
@@ -712,6 +823,7 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
     // z-marching form: 6-adjacency, two classes, intensities in [0, 32767] (bit 15 = seed)
     const bool march = rm.model == RELAX_INTENSITY && nadj == 7 && nc == 2 && wi.minI >= 0 && wi.maxI <= 0x7FFF &&
                        n < (1ll << 31) && niter > 0;
+    const bool march26 = nadj == 27 && nc == 2 && n < (1ll << 31);   // any weight model
     int16_t *pimg = nullptr;
     if (march) {
         pimg = buf<int16_t>(c, "relax.pimg", n);
@@ -747,6 +859,9 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
         } else if (nadj == 7) {
             if (i16) launch_iter<7, int16_t>(c, d_image, a, rm.model, fin, fout, seedcls, lo);
             else launch_iter<7, int32_t>(c, d_image, a, rm.model, fin, fout, seedcls, lo);
+        } else if (march26) {
+            if (i16) launch_march26<int16_t>(c, d_image, a, rm.model, mgrid, fin, fout, seedcls, lo);
+            else launch_march26<int32_t>(c, d_image, a, rm.model, mgrid, fin, fout, seedcls, lo);
         } else {
             if (i16) launch_iter<27, int16_t>(c, d_image, a, rm.model, fin, fout, seedcls, lo);
             else launch_iter<27, int32_t>(c, d_image, a, rm.model, fin, fout, seedcls, lo);
