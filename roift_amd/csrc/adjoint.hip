@@ -29,7 +29,34 @@ struct DiffArgs {
     R sigma;
     R alpha;
     R Dpq[27];
+    FastDiv fx, fy;   // volumes below 2^31 voxels: coordinates by multiply-high division
+    int fast;
 };
+
+// Coordinates of voxel p.
+template <typename R>
+__device__ inline void diff_coords(const DiffArgs<R> &a, long long p, int &x, int &y, int &z)
+{
+    if (a.fast) {
+        const uint32_t t = a.fx.div((uint32_t)p);
+        x = (int)((uint32_t)p - t * (uint32_t)a.d.xs);
+        z = (int)a.fy.div(t);
+        y = (int)(t - (uint32_t)z * (uint32_t)a.d.ys);
+    } else {
+        const long long xy = (long long)a.d.xs * a.d.ys;
+        x = (int)(p % a.d.xs);
+        y = (int)((p / a.d.xs) % a.d.ys);
+        z = (int)(p / xy);
+    }
+}
+
+// x / Dpq[k]: 6-adjacency has Dpq = 1, and dividing by 1 is exact, so it is skipped there.
+template <int NADJ, typename R>
+__device__ inline R over_dpq(R x, const DiffArgs<R> &a, int k)
+{
+    if constexpr (NADJ == 27) return x / a.Dpq[k];
+    else return x;
+}
 
 template <typename R>
 __device__ inline R rexp(R x)
@@ -46,7 +73,7 @@ __device__ inline R dweight(const IT *img, const float *scale, const DiffArgs<R>
     g = rexp<R>(-(d * d) * a.inv_s2);
     sg = d > (R)0 ? (R)1 : (d < (R)0 ? (R)-1 : (R)0);
     pi = (R)1 + a.alpha * sg * ((R)2 * fq - (R)1);
-    return (R)scale[q] * g * pi / a.Dpq[k];
+    return over_dpq<NADJ, R>((R)scale[q] * g * pi, a, k);
 }
 
 template <typename R>
@@ -80,7 +107,8 @@ __global__ __launch_bounds__(256) void k_diff_fwd(const IT *__restrict__ img, co
     for (long long p = gtid(); p < d.n; p += gstride()) {
         R v = fin[p];
         if (!seed[p]) {
-            const int x = (int)(p % d.xs), y = (int)((p / d.xs) % d.ys), z = (int)(p / xy);
+            int x, y, z;
+            diff_coords(a, p, x, y, z);
             R num = 0, den = 0;
 #pragma unroll
             for (int k = 1; k < NADJ; k++) {
@@ -131,7 +159,8 @@ __global__ __launch_bounds__(256) void k_diff_bwd1(const IT *__restrict__ img, c
         R cp = 0, self = 0;
         const R l = lam[p];
         if (!seed[p] && l != (R)0) {
-            const int x = (int)(p % d.xs), y = (int)((p / d.xs) % d.ys), z = (int)(p / xy);
+            int x, y, z;
+            diff_coords(a, p, x, y, z);
             R den = 0;
 #pragma unroll
             for (int k = 1; k < NADJ; k++) {
@@ -154,15 +183,17 @@ __global__ __launch_bounds__(256) void k_diff_bwd2(const IT *__restrict__ img, c
                                                     DiffArgs<R> a, const R *__restrict__ fin,
                                                     const R *__restrict__ fout, const R *__restrict__ c,
                                                     R *__restrict__ lam_out, R *__restrict__ gscale,
-                                                    double *__restrict__ gpar)
+                                                    double *__restrict__ part)
 {
     using A = Adjacency<NADJ>;
     const Dims d = a.d;
     const long long xy = (long long)d.xs * d.ys;
     double gsig = 0.0, galp = 0.0;
     const R s3 = a.sigma * a.sigma * a.sigma;
+    const double k2s3 = 2.0 / (double)s3;
     for (long long q = gtid(); q < d.n; q += gstride()) {
-        const int x = (int)(q % d.xs), y = (int)((q / d.xs) % d.ys), z = (int)(q / xy);
+        int x, y, z;
+        diff_coords(a, q, x, y, z);
         const R fq = fin[q];
         R lq = lam_out[q], gs = 0;
 #pragma unroll
@@ -176,21 +207,39 @@ __global__ __launch_bounds__(256) void k_diff_bwd2(const IT *__restrict__ img, c
             R g, sg, pi;
             const R w = dweight<NADJ, IT, R>(img, scale, a, k, p, q, fq, g, sg, pi);
             const R common = fq - fout[p];
-            const R dwdf = (R)scale[q] * g * a.alpha * sg * (R)2 / a.Dpq[k];
+            const R dwdf = over_dpq<NADJ, R>((R)scale[q] * g * a.alpha * sg * (R)2, a, k);
             lq += cp * (w + common * dwdf);
-            gs += cp * common * g * pi / a.Dpq[k];
+            gs += over_dpq<NADJ, R>(cp * common * g * pi, a, k);
             const R dd = (R)((int)img[p] - (int)img[q]);
-            gsig += (double)(cp * common) * (double)w * 2.0 * (double)dd * (double)dd / (double)s3;
-            galp += (double)(cp * common) * (double)((R)scale[q] * g * sg * ((R)2 * fq - (R)1) / a.Dpq[k]);
+            gsig += (double)(cp * common) * (double)w * (double)dd * (double)dd * k2s3;
+            galp += (double)(cp * common) * (double)over_dpq<NADJ, R>((R)scale[q] * g * sg * ((R)2 * fq - (R)1), a, k);
         }
         lam_out[q] = lq;
         gscale[q] += gs;
     }
+    // the block's sums join its own partial slots (the launches are ordered on one stream): one
+    // device atomic per block on two shared addresses serialised, about 200 of 220 us per launch
     gsig = block_reduce(gsig, OpAdd());
     galp = block_reduce(galp, OpAdd());
     if (threadIdx.x == 0) {
-        atomicAdd(&gpar[0], gsig);
-        atomicAdd(&gpar[1], galp);
+        part[2 * blockIdx.x] += gsig;
+        part[2 * blockIdx.x + 1] += galp;
+    }
+}
+
+// gpar[0..1] += the sums of the nb blocks' partial slots, in block order (one workgroup)
+__global__ __launch_bounds__(256) void k_diff_parts(const double *__restrict__ part, int nb, double *__restrict__ gpar)
+{
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) {
+        s0 += part[2 * i];
+        s1 += part[2 * i + 1];
+    }
+    s0 = block_reduce(s0, OpAdd());
+    s1 = block_reduce(s1, OpAdd());
+    if (threadIdx.x == 0) {
+        gpar[0] += s0;
+        gpar[1] += s1;
     }
 }
 
@@ -211,6 +260,8 @@ void diff_run(roift_ctx &c, const DiffArgs<R> &a, const void *d_image, const int
     R *lam = buf<R>(c, "diff.lam", n), *lam2 = buf<R>(c, "diff.lam2", n);
     R *cc = buf<R>(c, "diff.c", n);
     double *dacc = buf<double>(c, "diff.acc", 4);
+    double *part = buf<double>(c, "diff.part", 2 * (size_t)G);
+    fill<double>(c, part, 2 * (long long)G, 0.0);
     int *bad = reinterpret_cast<int *>(dacc + 3);
     const double zero[4] = {0, 0, 0, 0};
     ROIFT_HIP_CHECK(hipMemcpyAsync(dacc, zero, sizeof(zero), hipMemcpyHostToDevice, st));
@@ -241,10 +292,11 @@ void diff_run(roift_ctx &c, const DiffArgs<R> &a, const void *d_image, const int
             hipLaunchKernelGGL((k_diff_bwd1<NADJ, IT, R>), dim3(G), dim3(256), 0, st, img, d_scale, a, seed, fin,
                                lam, cc, lam2);
             hipLaunchKernelGGL((k_diff_bwd2<NADJ, IT, R>), dim3(G), dim3(256), 0, st, img, d_scale, a, fin, fout,
-                               cc, lam2, d_gscale, dacc + 1);
+                               cc, lam2, d_gscale, part);
             std::swap(lam, lam2);
         }
     }
+    hipLaunchKernelGGL(k_diff_parts, dim3(1), dim3(256), 0, st, part, (int)G, dacc + 1);
     ROIFT_HIP_CHECK(hipGetLastError());
     double h[4];
     ROIFT_HIP_CHECK(hipMemcpyAsync(h, dacc, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -266,6 +318,11 @@ void diff_dispatch(roift_ctx &c, int nadj, R sigma, R alpha, const Dims &d, cons
     a.sigma = sigma;
     a.inv_s2 = (R)1 / (sigma * sigma);
     a.alpha = alpha;
+    a.fast = d.n < (1ll << 31) ? 1 : 0;
+    if (a.fast) {
+        a.fx = FastDiv((uint32_t)d.xs);
+        a.fy = FastDiv((uint32_t)d.ys);
+    }
     for (int k = 0; k < 27; k++) a.Dpq[k] = 1;
     if (nadj == 27)
         for (int k = 1; k < 27; k++) {
