@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_diff_fwd(const IT *__restrict__ img, co
 // Loss and its gradient with respect to the final f.
 template <typename R>
 __global__ __launch_bounds__(256) void k_diff_loss(const R *__restrict__ f, const float *__restrict__ target,
-                                                    long long n, R eps, R *__restrict__ lam, double *__restrict__ loss)
+                                                    long long n, R eps, R *__restrict__ lam, double *__restrict__ part)
 {
     double acc = 0.0;
     for (long long p = gtid(); p < n; p += gstride()) {
@@ -143,7 +143,16 @@ __global__ __launch_bounds__(256) void k_diff_loss(const R *__restrict__ f, cons
         lam[p] = clamped ? (R)0 : (R)(((double)t / o - (1.0 - (double)t) / (1.0 - (double)o)) / (double)n);
     }
     acc = block_reduce(acc, OpAdd());
-    if (threadIdx.x == 0) atomicAdd(loss, acc / (double)n);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc / (double)n;   // summed by k_diff_sum
+}
+
+// out[0] += the sum of part[0 .. nb), in block order (one workgroup)
+__global__ __launch_bounds__(256) void k_diff_sum(const double *__restrict__ part, int nb, double *__restrict__ out)
+{
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) s += part[i];
+    s = block_reduce(s, OpAdd());
+    if (threadIdx.x == 0) out[0] += s;
 }
 
 template <int NADJ, typename IT, typename R>
@@ -277,7 +286,9 @@ void diff_run(roift_ctx &c, const DiffArgs<R> &a, const void *d_image, const int
         std::swap(cur, nxt);
     }
     if (d_memb) ROIFT_HIP_CHECK(hipMemcpyAsync(d_memb, cur, rb, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_diff_loss<R>, dim3(G), dim3(256), 0, st, cur, d_target, n, eps, lam, dacc);
+    double *lpart = buf<double>(c, "diff.lpart", (size_t)G);
+    hipLaunchKernelGGL(k_diff_loss<R>, dim3(G), dim3(256), 0, st, cur, d_target, n, eps, lam, lpart);
+    hipLaunchKernelGGL(k_diff_sum, dim3(1), dim3(256), 0, st, lpart, (int)G, dacc);
     fill<R>(c, d_gscale, n, (R)0);
     ROIFT_HIP_CHECK(hipGetLastError());
     // backward, segment by segment
