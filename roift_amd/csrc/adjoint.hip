@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void k_diff_bwd2(const IT *__restrict__ img, c
         gscale[q] += gs;
     }
     // the block's sums join its own partial slots (the launches are ordered on one stream): one
-    // device atomic per block on two shared addresses serialised, about 200 of 220 us per launch
+    // device atomic per block on two shared addresses serialised and took most of the pass
     gsig = block_reduce(gsig, OpAdd());
     galp = block_reduce(galp, OpAdd());
     if (threadIdx.x == 0) {

@@ -76,9 +76,9 @@ struct RelaxArgs {
 // (ORelax_1_Multi gft_ift.cpp:8800-8816; ORelax_1_Multi_GradWeight
 // exp_gradient_weight.cpp:407-422; ORelax_1_Multi_GaussRBF exp_gaussian_rbf_relax.cpp:343-345),
 // in the reference's FP32 operation order, except that the RBF weight multiplies by the
-// reciprocals of sigma^2 and Dpq instead of dividing (one rounding each way: the 384x384x256
-// 26-adjacency relaxation takes 105 instead of 130 ms; memberships stay within 1e-6 of the FP32
-// oracle and labels equal the reference's).  s is the soft polarity of q.
+// reciprocals of sigma^2 and Dpq instead of dividing (one rounding each way; the two IEEE
+// divisions were most of its instructions; memberships stay within 1e-6 of the FP32 oracle and
+// labels equal the reference's).  s is the soft polarity of q.
 template <int NADJ, int WM>
 __device__ inline float model_weight(const RelaxArgs &a, int k, int Ip, int Iq, long long p, long long q, float s)
 {
