@@ -262,7 +262,7 @@ __global__ void k_labels_from_init(long long n, const int32_t *__restrict__ labe
 __global__ void k_memb_out2(long long n, const float *__restrict__ f, int bg, float *__restrict__ out)
 {
     for (long long p = gtid(); p < n; p += gstride()) {
-        const float v = f[p];
+        const float v = fabsf(f[p]);   // the z-marching form keeps seeds' memberships negated
         out[bg * n + p] = v;
         out[(1 - bg) * n + p] = 1.0f - v;
     }
@@ -271,18 +271,20 @@ __global__ void k_memb_out2(long long n, const float *__restrict__ f, int bg, fl
 // ---------------------------------------------------------------- z-marching form
 // 6-adjacency, two classes: each thread owns one (x, y) column of a 64x4 tile and
 // marches a chunk of z planes, keeping the z-1 / z / z+1 values in registers and
-// reading its x/y neighbours (L1/L2-resident) directly.  The image is int16 with the
-// seed flag in bit 15 (intensities are 0..32767); seeds keep their membership.
+// reading its x/y neighbours (L1/L2-resident) directly.  The image is int16 (intensities
+// 0..32767); a seed's membership carries its sign bit (-f, -0 for 0), so that the stencil reads
+// |f| with free operand modifiers and seeds keep their membership.
 constexpr int MX = 64, MY = 4, MZ = 32, MY_MAX = 8;
 
 // Weight of neighbour q in p's update, restated for fewer operations:
 //   |d| * (1 +- s/100) = |d| + d*t  with d = I_p - I_q and t = s/100 = tA*f_q + tB
 // (the polarity's sign is the sign of d).  Rounding differs from the reference's
-// FP32 order by a few ulp; results stay within 1e-5 of the FP64 oracle.
-__device__ inline float relax_w8(float d, float fq, const RelaxArgs &a)
+// FP32 order by a few ulp; results stay within 1e-5 of the FP64 oracle.  wk is Wmax, or -inf
+// for a neighbour outside the volume (its weight is then +0).
+__device__ inline float relax_w8(float wk, float d, float fq, const RelaxArgs &a)
 {
     const float t = fmaf(a.tA, fq, a.tB);
-    float w = a.Wmax - fmaf(d, t, fabsf(d));
+    float w = wk - fmaf(d, t, fabsf(d));
     w = fmaxf(w, 0.0f);
     w = w * w;
     w = w * w;
@@ -341,90 +343,103 @@ __global__ __launch_bounds__(MX *MY_MAX) void k_relax2_march(const int16_t *__re
     const int z1 = min(zs, z0 + (BATCH ? MZ : a.mz));
     if (x >= xs || y >= ys || z0 >= zs) return;   // no block-level synchronisation below
     const uint32_t xy = (uint32_t)(xs * ys), uxs = (uint32_t)xs;
-    img += base;
-    fin += base;
-    fout += base;
-    if (label_out) label_out += base;
-    // 32-bit element offsets from a uniform base: the loads take a scalar base and a
-    // 32-bit vector offset, with no 64-bit address arithmetic per neighbour
-    auto LI = [&](uint32_t i) -> int { return *(const int16_t *)((const char *)img + (size_t)(i << 1)); };
-    auto LF = [&](uint32_t i) -> float { return *(const float *)((const char *)fin + (size_t)(i << 2)); };
-    uint32_t p = (uint32_t)x + (uint32_t)y * uxs + (uint32_t)z0 * xy;
+    // Plane z's arrays from uniform (scalar) base pointers that advance one plane per step, and
+    // loop-invariant 32-bit in-plane offsets: a load takes the scalar base and the vector offset
+    // as they are, with no per-step vector address arithmetic.
+    const size_t zoff = (size_t)base + (size_t)z0 * xy;
+    const int16_t *iz = img + zoff;
+    const float *fz = fin + zoff;
+    float *oz = fout + zoff;
+    int32_t *lz = label_out ? label_out + zoff : nullptr;
+    // loads and stores at 32-bit byte offsets from a base pointer
+    auto LI = [](const int16_t *b, uint32_t o) -> uint32_t { return *(const uint16_t *)((const char *)b + (size_t)o); };
+    auto LF = [](const float *b, uint32_t o) -> float { return *(const float *)((const char *)b + (size_t)o); };
+    // 2^23 + intensity as a float: one OR instead of a conversion; the difference of two such
+    // values is the exact integer difference (both lie in [2^23, 2^24))
+    auto BI = [](uint32_t r) -> float { return __uint_as_float(r | 0x4B000000u); };
     const int lab_bg = a.cls_label[a.bg], lab_ob = a.cls_label[1 - a.bg];
+    // The in-plane neighbours are loaded unconditionally, so that all of a voxel's loads are in
+    // flight together (loads under the bounds tests would each wait for their own data): an
+    // absent neighbour reads the voxel itself, which is in the volume, and its Wmax is -inf, so
+    // its weight max(-inf, 0)^8 is +0.  Adding a +0 weight and a +0 product leaves both sums
+    // bit-identical, since the weights and memberships are non-negative.
+    const uint32_t o0 = (uint32_t)x + (uint32_t)y * uxs;
+    const uint32_t oym = y > 0 ? o0 - uxs : o0, oxm = x > 0 ? o0 - 1u : o0, oxp = x < xs - 1 ? o0 + 1u : o0,
+                   oyp = y < ys - 1 ? o0 + uxs : o0;
+    // byte offsets of the column and its in-plane neighbours: int16 image (h*) and float (b*)
+    uint32_t h0 = o0 << 1, hym = oym << 1, hxm = oxm << 1, hxp = oxp << 1, hyp = oyp << 1;
+    uint32_t b0 = o0 << 2, bym = oym << 2, bxm = oxm << 2, bxp = oxp << 2, byp = oyp << 2;
+    const float ninf = -__builtin_huge_valf();
+    const float wym = y > 0 ? a.Wmax : ninf, wxm = x > 0 ? a.Wmax : ninf, wxp = x < xs - 1 ? a.Wmax : ninf,
+                wyp = y < ys - 1 ? a.Wmax : ninf;
     bool hm = z0 > 0;
-    float Im = 0.0f, fm = 0.0f;
+    float Bm = 0.0f, fm = 0.0f;
     if (hm) {
-        Im = (float)(LI(p - xy) & 0x7FFF);
-        fm = LF(p - xy);
+        Bm = BI(LI(iz - xy, h0));
+        fm = LF(fz - xy, b0);
     }
-    int rc = LI(p);
-    float fc = LF(p);
-    // The in-plane neighbours are loaded unconditionally, so that all of a voxel's loads are
-    // in flight together (loads under the bounds tests would each wait for their own data):
-    // an absent neighbour reads the voxel itself, which is in the volume, and its weight is
-    // zeroed.  Adding a +0 weight and a +0 product leaves both sums bit-identical, since the
-    // weights and memberships are non-negative.
-    const uint32_t kym = y > 0 ? ~0u : 0u, kxm = x > 0 ? ~0u : 0u, kxp = x < xs - 1 ? ~0u : 0u,
-                   kyp = y < ys - 1 ? ~0u : 0u;
-    const uint32_t dym = kym & uxs, dxm = kxm & 1u, dxp = kxp & 1u, dyp = kyp & uxs;
-    auto keep = [](float w, uint32_t k) { return __uint_as_float(__float_as_uint(w) & k); };
-    for (int z = z0; z < z1; z++, p += xy) {
+    uint32_t rc = LI(iz, h0);
+    float fc = LF(fz, b0);
+    for (int z = z0; z < z1; z++, iz += xy, fz += xy, oz += xy) {
+        // re-issued per step (no instruction), so that each load pairs a 32-bit offset with the
+        // scalar base instead of taking a 64-bit address hoisted out of the loop
+        asm volatile("" : "+v"(h0), "+v"(hym), "+v"(hxm), "+v"(hxp), "+v"(hyp));
+        asm volatile("" : "+v"(b0), "+v"(bym), "+v"(bxm), "+v"(bxp), "+v"(byp));
+        // plane z+1, or plane z again at the last plane (uniform choice; its weight is skipped)
         const bool hp = z + 1 < zs;
-        int rn = 0;
-        float fn = 0.0f;
-        if (hp) {
-            rn = LI(p + xy);
-            fn = LF(p + xy);
-        }
-        const int rym = LI(p - dym), rxm = LI(p - dxm), rxp = LI(p + dxp), ryp = LI(p + dyp);
-        const float fym = LF(p - dym), fxm = LF(p - dxm), fxp = LF(p + dxp), fyp = LF(p + dyp);
-        const float Ic = (float)(rc & 0x7FFF);
+        const uint32_t rn = LI(hp ? iz + xy : iz, h0);
+        const float fn = LF(hp ? fz + xy : fz, b0);
+        const uint32_t rym = LI(iz, hym), rxm = LI(iz, hxm), rxp = LI(iz, hxp), ryp = LI(iz, hyp);
+        const float fym = LF(fz, bym), fxm = LF(fz, bxm), fxp = LF(fz, bxp), fyp = LF(fz, byp);
+        const float Bc = BI(rc);
         float v = fc;
-        if (!(rc & 0x8000)) {
+        if (!(__float_as_uint(fc) >> 31)) {
             // neighbours in Spheric order: -z, -y, -x, +x, +y, +z (the reference's summation order)
             float sw = 0.0f, acc = 0.0f, w;
             if (hm) {
-                w = relax_w8(Ic - Im, fm, a);
+                w = relax_w8(a.Wmax, Bc - Bm, fabsf(fm), a);
                 if constexpr (DPQ) w /= a.Dpq[1];
                 sw += w;
-                acc = fmaf(w, fm, acc);
+                acc = fmaf(w, fabsf(fm), acc);
             }
-            w = keep(relax_w8(Ic - (float)(rym & 0x7FFF), fym, a), kym);
+            w = relax_w8(wym, Bc - BI(rym), fabsf(fym), a);
             if constexpr (DPQ) w /= a.Dpq[2];
             sw += w;
-            acc = fmaf(w, fym, acc);
-            w = keep(relax_w8(Ic - (float)(rxm & 0x7FFF), fxm, a), kxm);
+            acc = fmaf(w, fabsf(fym), acc);
+            w = relax_w8(wxm, Bc - BI(rxm), fabsf(fxm), a);
             if constexpr (DPQ) w /= a.Dpq[3];
             sw += w;
-            acc = fmaf(w, fxm, acc);
-            w = keep(relax_w8(Ic - (float)(rxp & 0x7FFF), fxp, a), kxp);
+            acc = fmaf(w, fabsf(fxm), acc);
+            w = relax_w8(wxp, Bc - BI(rxp), fabsf(fxp), a);
             if constexpr (DPQ) w /= a.Dpq[4];
             sw += w;
-            acc = fmaf(w, fxp, acc);
-            w = keep(relax_w8(Ic - (float)(ryp & 0x7FFF), fyp, a), kyp);
+            acc = fmaf(w, fabsf(fxp), acc);
+            w = relax_w8(wyp, Bc - BI(ryp), fabsf(fyp), a);
             if constexpr (DPQ) w /= a.Dpq[5];
             sw += w;
-            acc = fmaf(w, fyp, acc);
+            acc = fmaf(w, fabsf(fyp), acc);
             if (hp) {
-                const float In = (float)(rn & 0x7FFF);
-                w = relax_w8(Ic - In, fn, a);
+                w = relax_w8(a.Wmax, Bc - BI(rn), fabsf(fn), a);
                 if constexpr (DPQ) w /= a.Dpq[6];
                 sw += w;
-                acc = fmaf(w, fn, acc);
+                acc = fmaf(w, fabsf(fn), acc);
             }
             v = (sw > 0.0f) ? acc / sw : fc;
         }
-        *(float *)((char *)fout + (size_t)(p << 2)) = v;
-        if (label_out) *(int32_t *)((char *)label_out + (size_t)(p << 2)) = ((1.0f - v) > v) ? lab_ob : lab_bg;
+        *(float *)((char *)oz + (size_t)b0) = v;
+        if (lz) {
+            const float av = fabsf(v);
+            *(int32_t *)((char *)lz + (size_t)b0) = ((1.0f - av) > av) ? lab_ob : lab_bg;
+            lz += xy;
+        }
         hm = true;
-        Im = Ic;
+        Bm = Bc;
         fm = fc;
         rc = rn;
         fc = fn;
     }
 }
 
-// int16 image with the seed flag in bit 15, for the z-marching form
 // 26-adjacency, two classes, any weight model: each thread owns one (x, y) column of a 64 x my
 // tile and marches a chunk of z planes, keeping the 3 x 3 neighbourhood of planes z-1, z and z+1
 // in registers, so a voxel costs 9 image and 9 membership loads (plane z+1) instead of 26 + 26.
@@ -519,12 +534,16 @@ __global__ __launch_bounds__(MX *MY_MAX) void k_relax2_march26(const IT *__restr
     }
 }
 
+// int16 copy of the image for the z-marching form (intensities 0..32767), and the seeds' sign
+// bit on their initial membership
 template <typename IT>
 __global__ void k_pack_image(const IT *__restrict__ img, const uint8_t *__restrict__ seedcls, long long n,
-                             int16_t *__restrict__ out)
+                             int16_t *__restrict__ out, float *__restrict__ f)
 {
-    for (long long p = gtid(); p < n; p += gstride())
-        out[p] = (int16_t)((int)img[p] | (seedcls[p] != NOSEED ? 0x8000 : 0));
+    for (long long p = gtid(); p < n; p += gstride()) {
+        out[p] = (int16_t)img[p];
+        if (seedcls[p] != NOSEED) f[p] = -f[p];
+    }
 }
 
 template <int NADJ, typename IT, int WM>
@@ -710,12 +729,13 @@ __global__ void k_batch_pack(const IT *__restrict__ img, long long n, int16_t *_
     for (long long p = gtid(); p < n; p += gstride()) out[p] = (int16_t)((int)img[p] & 0x7FFF);
 }
 
-__global__ void k_batch_seedbits(const int32_t *__restrict__ S, int ns, long long n, int16_t *__restrict__ packed)
+// the seeds' sign bit on their initial membership (setting a bit: listing a seed twice is harmless)
+__global__ void k_batch_seedbits(const int32_t *__restrict__ S, int ns, long long n, float *__restrict__ f)
 {
     for (long long i = gtid(); i < ns; i += gstride()) {
         const int v = S[i];
         if (v < 0 || v >= n) continue;   // an invalid seed (reported by the caller's checks)
-        packed[v] = (int16_t)((int)packed[v] | 0x8000);
+        f[v] = __uint_as_float(__float_as_uint(f[v]) | 0x80000000u);
     }
 }
 
@@ -822,15 +842,17 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
     const bool i16 = image_dtype == ROIFT_I16;
     // z-marching form: 6-adjacency, two classes, intensities in [0, 32767] (bit 15 = seed)
     const bool march = rm.model == RELAX_INTENSITY && nadj == 7 && nc == 2 && wi.minI >= 0 && wi.maxI <= 0x7FFF &&
-                       n < (1ll << 31) && niter > 0;
+                       n < (1ll << 31) && (long long)d.xs * d.ys < (1ll << 30) && niter > 0;
     const bool march26 = nadj == 27 && nc == 2 && n < (1ll << 31);   // any weight model
     int16_t *pimg = nullptr;
     if (march) {
         pimg = buf<int16_t>(c, "relax.pimg", n);
         if (i16)
-            hipLaunchKernelGGL(k_pack_image<int16_t>, dim3(G), dim3(256), 0, st, (const int16_t *)d_image, seedcls, n, pimg);
+            hipLaunchKernelGGL(k_pack_image<int16_t>, dim3(G), dim3(256), 0, st, (const int16_t *)d_image, seedcls, n, pimg,
+                               f0);
         else
-            hipLaunchKernelGGL(k_pack_image<int32_t>, dim3(G), dim3(256), 0, st, (const int32_t *)d_image, seedcls, n, pimg);
+            hipLaunchKernelGGL(k_pack_image<int32_t>, dim3(G), dim3(256), 0, st, (const int32_t *)d_image, seedcls, n, pimg,
+                               f0);
         ROIFT_HIP_CHECK(hipGetLastError());
     }
     // tile shape: ROIFT_PARAM_RELAX_TILE = rows | planes << 8 | xcd << 16 (0: 4 rows, 32 planes, XCD-grouped)
@@ -1015,7 +1037,7 @@ int run_relax_batch(roift_ctx &c, const std::vector<BatchVolume> &vols, const vo
     else
         hipLaunchKernelGGL(k_batch_pack<int32_t>, dim3(G), dim3(256), 0, st, (const int32_t *)d_image, n, pimg);
     if (n_seeds > 0)
-        hipLaunchKernelGGL(k_batch_seedbits, dim3(grid_for(n_seeds)), dim3(256), 0, st, d_seeds, n_seeds, n, pimg);
+        hipLaunchKernelGGL(k_batch_seedbits, dim3(grid_for(n_seeds)), dim3(256), 0, st, d_seeds, n_seeds, n, f0);
     ROIFT_HIP_CHECK(hipGetLastError());
     const unsigned nt = (unsigned)tiles.size();
     if (!capture) ev_record(c, 12);
