@@ -19,6 +19,7 @@
 // (2 B int16) + membership read (4 B/channel) + membership write (4 B/channel).
 #include <algorithm>
 #include <cstdio>
+#include <type_traits>
 
 #include "ctx.hpp"
 
@@ -66,6 +67,10 @@ struct RelaxArgs {
     float alpha;
     float sigma2;
     float inv_sigma2, inv_Dpq[27];   // RBF model: 1 / sigma2, 1 / Dpq[k]
+    // RBF model, 26-adjacency march: exp(-d^2 / sigma2) by |d| for d < rbf_T (the last entry is 0,
+    // as is every larger |d|); rbf_T = 0 when no table is used
+    const float *rbf_tab;
+    int rbf_T;
     // z-marching tiles: rows (blockDim.y) and planes per workgroup; xcd = 1 maps consecutive
     // tiles to one XCD (workgroups are dealt to the 8 XCDs round-robin), so the tiles that
     // share halo rows share an L2
@@ -447,12 +452,28 @@ __global__ __launch_bounds__(MX *MY_MAX) void k_relax2_march(const int16_t *__re
 // weight and a +0 product leave both sums bit-identical to skipping it (weights and memberships
 // are >= 0), so the result equals k_relax2's, summed in the reference's Spheric order (the
 // raster order of the 3 x 3 x 3 block without its centre).
+// RBF weights from a table: the model's exp(-d^2 / sigma2) depends on the integer |d| alone, so
+// for intensities in [0, 32767] one LDS read by |d| (from v_sad_u16) replaces the exponential,
+// bit-identical to it; |d| >= rbf_T - 1 reads the table's last entry, 0, as the exponential of
+// anything below -105 is.  Absent columns add 2^30 to |d|, so they read the 0 as well.
+constexpr int RELAX_RBF_TAB = 3;
+constexpr int RBF_TMAX = 4096;
+
+__global__ void k_rbf_table(float *__restrict__ tab, int T, float inv_sigma2)
+{
+    for (int d = (int)gtid(); d < T; d += (int)gstride()) {
+        const float diff = (float)d;   // model_weight's expression
+        tab[d] = expf(-(diff * diff) * inv_sigma2);
+    }
+}
+
 template <typename IT, int WM>
 __global__ __launch_bounds__(MX *MY_MAX) void k_relax2_march26(const IT *__restrict__ img, RelaxArgs a,
                                                             const float *__restrict__ fin, float *__restrict__ fout,
                                                             const uint8_t *__restrict__ seedcls,
                                                             int32_t *__restrict__ label_out)
 {
+    constexpr bool TAB = WM == RELAX_RBF_TAB;
     const int xs = a.d.xs, ys = a.d.ys, zs = a.d.zs;
     const unsigned ntx = (unsigned)(xs + MX - 1) / MX, nty = (unsigned)(ys + a.my - 1) / a.my;
     const unsigned ntiles = ntx * nty * ((unsigned)(zs + a.mz - 1) / a.mz);
@@ -463,6 +484,11 @@ __global__ __launch_bounds__(MX *MY_MAX) void k_relax2_march26(const IT *__restr
     const int y = (int)((tile / ntx) % nty) * a.my + (int)threadIdx.y;
     const int z0 = (int)(tile / (ntx * nty)) * a.mz;
     const int z1 = min(zs, z0 + a.mz);
+    __shared__ float tab[TAB ? RBF_TMAX : 1];
+    if constexpr (TAB) {   // every thread of the workgroup is still here
+        for (int i = (int)(threadIdx.x + threadIdx.y * MX); i < a.rbf_T; i += MX * (int)blockDim.y) tab[i] = a.rbf_tab[i];
+        __syncthreads();
+    }
     if (x >= xs || y >= ys || z0 >= zs) return;   // no block-level synchronisation below
     const uint32_t uxs = (uint32_t)xs, xy = (uint32_t)xs * (uint32_t)ys;
     // the 3 x 3 columns around (x, y): c = (dy + 1) * 3 + (dx + 1); absent ones read column 4
@@ -477,60 +503,106 @@ __global__ __launch_bounds__(MX *MY_MAX) void k_relax2_march26(const IT *__restr
     }
     int I[3][9] = {};
     float F[3][9] = {};
-    const uint32_t pc = (uint32_t)x + (uint32_t)y * uxs;   // the column's voxel in plane 0
-    auto load_plane = [&](int slot, int z) {
-        const uint32_t b = pc + (uint32_t)z * xy;
+    const uint32_t pc = (uint32_t)x + (uint32_t)y * uxs;   // the column's in-plane offset
+    // Byte offsets of the 3 x 3 columns (image and memberships), loop-invariant; plane z's arrays
+    // come from uniform (scalar) base pointers, so a load takes the scalar base and the vector
+    // offset with no per-step address arithmetic.
+    uint32_t hof[9], bof[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+        hof[c] = (uint32_t)((int)pc + off[c]) * (uint32_t)sizeof(IT);
+        bof[c] = (uint32_t)((int)pc + off[c]) * 4u;
+    }
+    uint32_t pb = pc * 4u, p1 = pc;
+    // the planes' register slots rotate with the unrolled steps below, not by copies
+    auto load_plane = [&](auto S, int z) {
+        constexpr int sl = decltype(S)::value;
+        const IT *ib = img + (size_t)z * xy;
+        const float *fb = fin + (size_t)z * xy;
 #pragma unroll
         for (int c = 0; c < 9; c++) {
-            I[slot][c] = (int)img[(uint32_t)((int)b + off[c])];
-            F[slot][c] = fin[(uint32_t)((int)b + off[c])];
+            I[sl][c] = (int)*(const IT *)((const char *)ib + (size_t)hof[c]);
+            F[sl][c] = *(const float *)((const char *)fb + (size_t)bof[c]);
         }
     };
-    if (z0 > 0) load_plane(0, z0 - 1);
-    load_plane(1, z0);
+    using S0 = std::integral_constant<int, 0>;
+    using S1 = std::integral_constant<int, 1>;
+    using S2 = std::integral_constant<int, 2>;
+    if (z0 > 0) load_plane(S0{}, z0 - 1);
+    load_plane(S1{}, z0);
     const int lab_bg = a.cls_label[a.bg], lab_ob = a.cls_label[1 - a.bg];
-    for (int z = z0; z < z1; z++) {
+    // one voxel: planes z-1, z and z+1 sit in slots A, B and C
+    auto step = [&](auto SA, auto SB, auto SC, int z) {
+        constexpr int A = decltype(SA)::value, B = decltype(SB)::value;
+        // re-issued per step (no instruction), so that each access pairs a 32-bit offset with
+        // the scalar base instead of taking a 64-bit address hoisted out of the loop
+        asm volatile("" : "+v"(hof[0]), "+v"(hof[1]), "+v"(hof[2]), "+v"(hof[3]), "+v"(hof[4]), "+v"(hof[5]),
+                     "+v"(hof[6]), "+v"(hof[7]), "+v"(hof[8]));
+        asm volatile("" : "+v"(bof[0]), "+v"(bof[1]), "+v"(bof[2]), "+v"(bof[3]), "+v"(bof[4]), "+v"(bof[5]),
+                     "+v"(bof[6]), "+v"(bof[7]), "+v"(bof[8]), "+v"(pb), "+v"(p1));
         const bool hm = z > 0, hp = z + 1 < zs;
-        if (hp) load_plane(2, z + 1);
+        if (hp) load_plane(SC, z + 1);
+        const size_t zo = (size_t)z * xy;
         const uint32_t p = pc + (uint32_t)z * xy;
-        const uint8_t sc = seedcls[p];
+        const uint8_t sc = *(seedcls + zo + p1);
         float v;
         if (sc != NOSEED) {
             v = (sc == a.bg) ? 1.0f : 0.0f;
+        } else if constexpr (TAB) {
+            // the same sums in the same order; an absent plane's terms (+0) are skipped
+            const uint32_t lim = (uint32_t)a.rbf_T - 1u, Ip = (uint32_t)I[B][4];
+            float sw = 0.0f, acc = 0.0f;
+#pragma unroll
+            for (int pl = 0; pl < 3; pl++) {
+                if ((pl == 0 && !hm) || (pl == 2 && !hp)) continue;   // uniform
+                const int sl = pl == 0 ? A : (pl == 1 ? B : decltype(SC)::value);
+#pragma unroll
+                for (int c = 0; c < 9; c++) {
+                    if (pl == 1 && c == 4) continue;
+                    const int j = pl * 9 + c, k = j < 13 ? j + 1 : j;   // Spheric index
+                    const uint32_t ad = __builtin_amdgcn_sad_u16(Ip, (uint32_t)I[sl][c], msk[c] ? 0u : 0x40000000u);
+                    const float w = tab[min(ad, lim)] * a.inv_Dpq[k];
+                    sw += w;
+                    acc += w * F[sl][c];
+                }
+            }
+            v = (sw > 0.0f) ? acc / sw : F[B][4];
         } else {
-            const int Ip = I[1][4];
+            const int Ip = I[B][4];
             float sw = 0.0f, acc = 0.0f;
             int k = 1;
 #pragma unroll
             for (int pl = 0; pl < 3; pl++) {
                 const uint32_t pm = (pl == 0 ? hm : (pl == 2 ? hp : true)) ? ~0u : 0u;
+                const int sl = pl == 0 ? A : (pl == 1 ? B : decltype(SC)::value);
 #pragma unroll
                 for (int c = 0; c < 9; c++) {
                     if (pl == 1 && c == 4) continue;
                     const uint32_t m = msk[c] & pm;
-                    const float fq = F[pl][c];
+                    const float fq = F[sl][c];
                     const long long q = m ? (long long)((int)p + off[c] + (pl - 1) * (int)xy) : (long long)p;
                     // soft polarity from the neighbour's memberships (gft_ift.cpp:8800-8802, 9077-9079)
                     const float s = a.use_pc ? (a.pcm[a.bg] * fq + a.pcm[1 - a.bg] * (1.0f - fq))
                                              : a.per * (2.0f * fq - 1.0f);
-                    float w = model_weight<27, WM>(a, k, Ip, I[pl][c], p, q, s);
+                    float w = model_weight<27, WM>(a, k, Ip, I[sl][c], p, q, s);
                     w = __uint_as_float(__float_as_uint(w) & m);
                     sw += w;
                     acc += w * __uint_as_float(__float_as_uint(fq) & m);
                     k++;
                 }
             }
-            v = (sw > 0.0f) ? acc / sw : F[1][4];
+            v = (sw > 0.0f) ? acc / sw : F[B][4];
         }
-        fout[p] = v;
-        if (label_out) label_out[p] = ((1.0f - v) > v) ? lab_ob : lab_bg;
-#pragma unroll
-        for (int c = 0; c < 9; c++) {
-            I[0][c] = I[1][c];
-            F[0][c] = F[1][c];
-            I[1][c] = I[2][c];
-            F[1][c] = F[2][c];
-        }
+        *(float *)((char *)(fout + zo) + (size_t)pb) = v;
+        if (label_out) *(int32_t *)((char *)(label_out + zo) + (size_t)pb) = ((1.0f - v) > v) ? lab_ob : lab_bg;
+    };
+    for (int z = z0; z < z1;) {
+        step(S0{}, S1{}, S2{}, z);
+        if (++z >= z1) break;
+        step(S1{}, S2{}, S0{}, z);
+        if (++z >= z1) break;
+        step(S2{}, S0{}, S1{}, z);
+        ++z;
     }
 }
 
@@ -577,6 +649,9 @@ void launch_march26(roift_ctx &c, const void *img, const RelaxArgs &a, int model
     if (model == RELAX_GRAD)
         hipLaunchKernelGGL((k_relax2_march26<IT, RELAX_GRAD>), grid, blk, 0, c.stream, (const IT *)img, a, fin, fout,
                            seedcls, label_out);
+    else if (model == RELAX_RBF && a.rbf_T > 0)
+        hipLaunchKernelGGL((k_relax2_march26<IT, RELAX_RBF_TAB>), grid, blk, 0, c.stream, (const IT *)img, a, fin,
+                           fout, seedcls, label_out);
     else if (model == RELAX_RBF)
         hipLaunchKernelGGL((k_relax2_march26<IT, RELAX_RBF>), grid, blk, 0, c.stream, (const IT *)img, a, fin, fout,
                            seedcls, label_out);
@@ -844,6 +919,25 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
     const bool march = rm.model == RELAX_INTENSITY && nadj == 7 && nc == 2 && wi.minI >= 0 && wi.maxI <= 0x7FFF &&
                        n < (1ll << 31) && (long long)d.xs * d.ys < (1ll << 30) && niter > 0;
     const bool march26 = nadj == 27 && nc == 2 && n < (1ll << 31);   // any weight model
+    if (march26 && rm.model == RELAX_RBF && wi.minI >= 0 && wi.maxI <= 0x7FFF) {
+        // the first |d| whose exponent is below -105 (float products, as the kernels form them):
+        // it and every larger |d| give exactly 0
+        int T = 0;
+        for (int dd = 0; dd < RBF_TMAX; dd++) {
+            const float fd = (float)dd;
+            if ((fd * fd) * a.inv_sigma2 >= 105.0f) {
+                T = dd + 1;
+                break;
+            }
+        }
+        if (T > 0) {
+            float *tab = buf<float>(c, "relax.rbftab", RBF_TMAX);
+            hipLaunchKernelGGL(k_rbf_table, dim3(grid_for(T)), dim3(256), 0, st, tab, T, a.inv_sigma2);
+            ROIFT_HIP_CHECK(hipGetLastError());
+            a.rbf_tab = tab;
+            a.rbf_T = T;
+        }
+    }
     int16_t *pimg = nullptr;
     if (march) {
         pimg = buf<int16_t>(c, "relax.pimg", n);
