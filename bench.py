@@ -47,8 +47,8 @@ LIMITED_BY = {
     "k_oift_persistent": "latency: ~2,100 grid barriers (2.4 us of atomics plus 1.6-3.5 us of agent-scope "
                          "L2 write-back/invalidate each) and the flood's dependent chains (1.7 us per "
                          "single-wave hop); HBM traffic is a by-product of the barriers' write-back/invalidate",
-    "k_relax2_march": "VALU: 94% busy, ~122 lane-instructions per voxel (the eighth power, the division, "
-                      "six neighbours)",
+    "k_relax2_march": "VALU and HBM together: 86% VALU-busy at ~88 lane-instructions per voxel (the eighth "
+                      "power, the division, six neighbours) with ~4 TB/s of algorithmic bytes",
     "k_weights": "HBM",
     "k_prep": "HBM (stores): 69 of its 75 bytes per voxel are writes",
     "k_blur": "VALU: the reference's FP32 tap order, ~50 instructions per output voxel",
