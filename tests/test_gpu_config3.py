@@ -148,3 +148,42 @@ def test_config3_full_size_rbf_relax_invariants(ctx, case3):
     assert (flat[sl, c.seeds] == 1.0).all() and (flat[1 - sl, c.seeds] == 0.0).all()
     want = np.where(m[1] > m[0], 1, 0)
     assert np.array_equal(lab.cpu().numpy(), want)
+
+
+def test_rbf_relax_table_and_exponential_paths_agree(ctx):
+    """The 26-adjacency relaxation reads the Gaussian-RBF weight from a table by |I_p - I_q| when
+    the intensities lie in [0, 32767] and the table fits (sigma up to about 400), and evaluates
+    the exponential otherwise.  The weight depends on the intensity difference alone, so the same
+    image shifted below 0 (which takes the exponential path) gives bit-identical memberships and
+    labels; a sigma too large for the table is checked against the oracles (within 1e-5 of both:
+    with sigma = 1000 no weight underflows, and the products by 1/sigma^2 and 1/Dpq that replace
+    the reference's divisions move memberships by about 1e-6 there)."""
+    import torch
+    c = P.multi_object_case(shape=(32, 48, 48), seed=5)
+    img = torch.from_numpy(c.image.astype(np.int16)).cuda()
+    seeds = torch.from_numpy(c.seeds.astype(np.int32)).cuda()
+    lab = torch.from_numpy(c.label_init()).cuda()
+    ctx.oift_fmax(img, seeds, lab, 1.75, c.per)
+    oift = lab.clone()
+    shifted = (img.to(torch.int32) - (int(c.image.max()) + 1)).to(torch.int16)
+    assert int(shifted.max()) < 0
+    niter = 20
+    for sigma in (0.8, 30.0):
+        la, lb = oift.clone(), oift.clone()
+        ma, _ = ctx.relax_rbf(img, seeds, la, niter, sigma, 1.75)
+        mb, _ = ctx.relax_rbf(shifted, seeds, lb, niter, sigma, 1.75)
+        print("rbf paths sigma", sigma, flush=True)
+        assert torch.equal(ma.view(torch.int32), mb.view(torch.int32))
+        assert torch.equal(la, lb)
+    sigma = 1000.0   # the table would need over 10,000 entries
+    lc = oift.clone()
+    mc, _ = ctx.relax_rbf(img, seeds, lc, niter, sigma, 1.75)
+    m = mc.cpu().numpy().astype(np.float64)
+    r32 = O.relax_model(c.image, c.seeds, oift.cpu().numpy(), niter, "rbf", sigma=sigma, adj_r=1.75, fp64=False)
+    r64 = O.relax_model(c.image, c.seeds, oift.cpu().numpy(), niter, "rbf", sigma=sigma, adj_r=1.75, fp64=True)
+    print("sigma 1000: max |err| vs FP32", float(np.abs(m - r32["memb"]).max()), "vs FP64",
+          float(np.abs(m - r64["memb"]).max()), flush=True)
+    assert np.abs(m - r32["memb"]).max() <= 1e-5
+    assert np.abs(m - r64["memb"]).max() <= 1e-5
+    near = np.abs(m[1] - 0.5) <= 1e-5
+    assert np.array_equal(lc.cpu().numpy()[~near], r32["label"][~near])
