@@ -918,7 +918,8 @@ int run_relax(roift_ctx &c, const Dims &d, int nadj, const void *d_image, int im
     // z-marching form: 6-adjacency, two classes, intensities in [0, 32767] (bit 15 = seed)
     const bool march = rm.model == RELAX_INTENSITY && nadj == 7 && nc == 2 && wi.minI >= 0 && wi.maxI <= 0x7FFF &&
                        n < (1ll << 31) && (long long)d.xs * d.ys < (1ll << 30) && niter > 0;
-    const bool march26 = nadj == 27 && nc == 2 && n < (1ll << 31);   // any weight model
+    // any weight model; 32-bit byte offsets within a plane
+    const bool march26 = nadj == 27 && nc == 2 && n < (1ll << 31) && (long long)d.xs * d.ys < (1ll << 30);
     if (march26 && rm.model == RELAX_RBF && wi.minI >= 0 && wi.maxI <= 0x7FFF) {
         // the first |d| whose exponent is below -105 (float products, as the kernels form them):
         // it and every larger |d| give exactly 0
