@@ -58,7 +58,8 @@ def test_relax_matches_reference_golden(ctx, case):
 @pytest.mark.parametrize("shape,seed,adj,per,niter", [((12, 13, 14), 0, 1.0, 50.0, 10), ((9, 21, 17), 1, 1.0, -50.0, 25),
                                                        ((11, 12, 13), 2, 1.75, 50.0, 12), ((1, 40, 50), 3, 1.0, 50.0, 30),
                                                        ((14, 14, 14), 4, 1.0, 0.0, 5), ((10, 10, 10), 5, 1.0, 50.0, 0),
-                                                       ((10, 12, 9), 6, 1.0, 100.0, 1)])
+                                                       ((10, 12, 9), 6, 1.0, 100.0, 1), ((31, 1, 45), 9, 1.0, 50.0, 20),
+                                                       ((17, 23, 1), 10, 1.0, -50.0, 20), ((1, 30, 40), 11, 1.75, 50.0, 15)])
 def test_relax_matches_fp64_oracle(ctx, shape, seed, adj, per, niter):
     c = P.small_case(shape, seed, per=per, adj_r=adj)
     o = O.oift_image(c.image, c.seeds, c.label_init(), adj, per)
